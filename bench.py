@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the UMAP fit path on one MI355X.
+
+Workload (plain UMAP, 1 GPU): n = 1,000,000 points, D = 64, a 50-component
+Gaussian mixture (means ~ N(0, 5^2 I), per-component std log-uniform in
+[0.5, 2]); exact Euclidean kNN k = 15 computed on the GPU (self at slot 0);
+init uniform [-10, 10]; 2-D; min_dist 0.1, spread 1, negative_sample_rate 5;
+500 epochs.
+
+A step is one full SGD run (Optimizer::step_epochs(500)) on a fresh optimizer
+made from the resident manifold; the graph is built once (its time is reported
+as graph_build_s).  value = positive edge-updates per second of the timed
+steps (an edge-update is one activation of optimize_layout_euclidean.rs:277-344
+with its negative samples; the count is the RNG-free f64 schedule).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "umap-rs_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BYTES_PER_UPDATE_D2 = 80  # 8 (head+tail u32) + 16*d (H, T read+write) + 4*d*r (negatives), d=2, r=5
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_data(n: int, dim: int, n_comp: int, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    means = rng.normal(0.0, 5.0, (n_comp, dim)).astype(np.float32)
+    stds = np.exp(rng.uniform(np.log(0.5), np.log(2.0), n_comp)).astype(np.float32)
+    lab = rng.integers(0, n_comp, n)
+    x = np.empty((n, dim), np.float32)
+    chunk = 1 << 18
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        z = rng.standard_normal((e - s, dim), dtype=np.float32)
+        x[s:e] = means[lab[s:e]] + z * stds[lab[s:e], None]
+    return x, lab
+
+
+def bytes_per_update(dim: int, nsr: int) -> int:
+    return 8 + 16 * dim + 4 * dim * nsr
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=64)
+    ap.add_argument("--k", type=int, default=15)
+    ap.add_argument("--epochs", type=int, default=500)
+    ap.add_argument("--mode", choices=["hogwild", "hogwild_racy"], default="hogwild")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's SGD sample")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-quality", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import umap_amd as U
+    if U.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+
+    n, dim, k, epochs = args.n, args.dim, args.k, args.epochs
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = k
+    cfg.manifold.min_dist = 0.1
+    cfg.optimization.n_epochs = epochs
+
+    t = time.perf_counter()
+    x, _ = make_data(n, dim, 50, seed=0)
+    init = np.random.default_rng(1).uniform(-10, 10, (n, 2)).astype(np.float32)
+    log(f"data n={n} D={dim} generated in {time.perf_counter() - t:.1f}s")
+
+    dx = U.DeviceArray.from_host(x)
+    U.synchronize()
+    t = time.perf_counter()
+    d_idx, d_dst = U.knn_bruteforce_device(dx, k)
+    U.synchronize()
+    knn_s = time.perf_counter() - t
+    log(f"exact kNN k={k} on GPU: {knn_s:.2f}s")
+    dx.free()
+
+    # graph build (learn_manifold from resident kNN)
+    U.synchronize()
+    t = time.perf_counter()
+    manifold = U.learn_manifold_device(cfg, d_idx, d_dst)
+    U.synchronize()
+    graph_s = time.perf_counter() - t
+    gt = manifold.timings()
+    log(f"learn_manifold: {graph_s:.3f}s wall, nnz={gt['nnz']} (directed {gt['nnz_directed']})")
+    d_init = U.DeviceArray.from_host(init)
+    U.synchronize()
+
+    mode = U.SGD_HOGWILD if args.mode == "hogwild" else U.SGD_HOGWILD_RACY
+    times, kernel_ms, updates, negs, prep_ms = [], [], [], [], []
+    last_opt = None
+    for it in range(args.warmup + args.steps):
+        m = manifold.clone()
+        U.synchronize()
+        t0 = time.perf_counter()
+        opt = U.Optimizer.new_device(m, d_init.ptr, n, 2, epochs, cfg, U.MetricType.Euclidean, seed=2 + it)
+        U.synchronize()
+        t1 = time.perf_counter()
+        opt.set_mode(mode)
+        U.synchronize()
+        t2 = time.perf_counter()
+        opt.step_epochs(epochs)
+        U.synchronize()
+        t3 = time.perf_counter()
+        p, q, ms = opt.stats()
+        tag = "warmup" if it < args.warmup else "timed"
+        log(f"step {it} ({tag}): optimizer_new {1e3 * (t1 - t0):.1f} ms, step_epochs({epochs}) "
+            f"{1e3 * (t3 - t2):.1f} ms wall / {ms:.1f} ms events, updates={p} negatives={q}")
+        if it >= args.warmup:
+            times.append(t3 - t2)
+            kernel_ms.append(ms)
+            updates.append(p)
+            negs.append(q)
+            prep_ms.append(1e3 * (t1 - t0))
+        if last_opt is not None:
+            del last_opt
+        last_opt = opt
+    n_edges = last_opt.info()[0]
+
+    total_t = float(sum(times))
+    value = float(sum(updates)) / total_t
+    r_neg = float(sum(negs)) / max(float(sum(updates)), 1.0)
+    bpu = bytes_per_update(2, cfg.optimization.negative_sample_rate)
+    kernel_s = float(sum(kernel_ms)) / 1e3
+    achieved = bpu * float(sum(updates)) / kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "sgd_rows_kernel<2,u16,atomic>" if args.mode == "hogwild" else "sgd_rows_kernel<2,u16,racy>",
+                "bytes_per_update": bpu, "avg_launch_us": round(1e6 * kernel_s / (epochs * len(kernel_ms)), 2),
+                "updates_per_launch": round(float(sum(updates)) / (epochs * len(kernel_ms)), 1)}
+    traffic_file = os.path.join(REPO, "profiles", "sgd_traffic_c2.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "SGD edge-updates/s + wall-to-target-loss, learned-init vs plain, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total_t / len(times), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixture; exact kNN computed on the GPU)",
+        "config": {"workload": f"plain UMAP fit path: n={n:,} D={dim} 50-component GMM, exact kNN k={k}, "
+                               f"2-D, min_dist 0.1, {epochs} epochs, Hogwild SGD",
+                   "n": n, "D": dim, "k": k, "epochs": epochs, "sgd_mode": args.mode,
+                   "edges_after_filter": int(n_edges), "graph_nnz": int(gt["nnz"])},
+        "roofline": roofline,
+        "graph_build_s": round(graph_s, 4),
+        "graph_stages_ms": {s["name"]: round(s["ms"], 3) for s in gt["stages"]},
+        "knn_s": round(knn_s, 3),
+        "optimizer_new_ms": round(float(np.mean(prep_ms)), 3),
+        "positive_updates_per_step": int(updates[0]),
+        "negatives_per_update": round(r_neg, 4),
+    }
+
+    if not args.skip_quality:
+        import umap_eval as E
+        g = last_opt.manifold().graph()
+        a, b = last_opt.manifold().curve_params()
+        y = last_opt.embedding()
+        t = time.perf_counter()
+        out["final_fuzzy_ce"] = round(E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), y, a, b), 1)
+        log(f"final fuzzy CE {out['final_fuzzy_ce']:.4g} ({time.perf_counter() - t:.1f}s)")
+
+    if not args.skip_cpu:
+        import oracle as O
+        st = last_opt.manifold().to_state()
+        om = O.Manifold(st["indptr"], st["indices"], st["data"], st["sigmas"], st["rhos"], n, st["a"], st["b"])
+        ost = O.OptimizerState(om, init, epochs, seed=2)
+        threads = O.num_threads()
+        # time one epoch, then run as many as fit the bound (epochs 1.. of 500;
+        # epoch 0 samples nothing)
+        ost.step_epochs(1, "hogwild")
+        done = 1
+        t = time.perf_counter()
+        ost.step_epochs(1, "hogwild")
+        one = time.perf_counter() - t
+        done += 1
+        more = int(max(0, min(epochs - done, args.cpu_seconds / max(one, 1e-6) - 1)))
+        c0 = int(ost.counts[0])
+        t = time.perf_counter()
+        if more:
+            ost.step_epochs(more, "hogwild")
+        dt = time.perf_counter() - t
+        cpu_updates = int(ost.counts[0]) - c0
+        cpu_rate = cpu_updates / dt if more else float(ost.counts[0]) / one
+        out["cpu_baseline"] = {"value": round(cpu_rate, 1), "unit": "edge-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"same graph and init, Hogwild epochs {done}..{done + more - 1} of {epochs} "
+                                         f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
+        log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

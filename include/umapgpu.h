@@ -1,0 +1,263 @@
+/*
+ * umapgpu.h -- C ABI of libumapgpu, the MI355X (gfx950) implementation of the
+ * umap-rs fit path: Umap::learn_manifold -> Optimizer::new ->
+ * Optimizer::step_epochs -> Optimizer::into_fitted, plus the Metric API.
+ *
+ * Every entry point below replaces one item of the reference crate (umap-rs
+ * 0.4.5); the citation names the reference file:line it stands in for.  A
+ * host facade in the caller's language rebuilds the reference signatures on
+ * top of these calls (umap-rs_amd/umap.hpp for C++, umap-rs_amd/umap_amd.py
+ * for Python).
+ *
+ * Conventions
+ *   - Plain pointers and sizes; host pointers unless the name ends in _device.
+ *   - Every call returns umapgpu_status.  The reference panics instead of
+ *     returning errors; here the panic text (identical wording) is available
+ *     from umapgpu_last_error() and the call leaves its outputs untouched.
+ *   - Calls are synchronous: when one returns, its device work is complete.
+ *   - Handles are not thread-safe; distinct handles may be used from distinct
+ *     threads.  "Consumed" handle arguments are invalid after a successful
+ *     call and must not be destroyed by the caller.
+ */
+#ifndef UMAPGPU_H
+#define UMAPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UMAPGPU_API __attribute__((visibility("default")))
+
+typedef enum umapgpu_status {
+  UMAPGPU_OK = 0,
+  UMAPGPU_INVALID_ARGUMENT = 1, /* a reference panic (validation, shapes, limits) */
+  UMAPGPU_OUT_OF_MEMORY = 2,
+  UMAPGPU_HIP_ERROR = 3,
+  UMAPGPU_OVERSTEP = 4, /* Optimizer::step_epochs past total_epochs (optimizer.rs:298-304) */
+  UMAPGPU_NO_DEVICE = 5,
+  UMAPGPU_UNSUPPORTED = 6
+} umapgpu_status;
+
+/* MetricType (metric.rs:7-12). */
+typedef enum umapgpu_metric_type { UMAPGPU_METRIC_EUCLIDEAN = 0, UMAPGPU_METRIC_GENERIC = 1 } umapgpu_metric_type;
+
+/* SGD execution modes of Optimizer::step_epochs (an addition: the reference
+ * always runs its Rayon Hogwild loop, optimizer.rs:337).
+ *   HOGWILD: one lane per head row walks its edges in order; tail-snapshot
+ *            updates use hardware f32 atomics (no lost updates).
+ *   HOGWILD_RACY: as HOGWILD with plain racy tail stores, as the reference.
+ *   SERIAL: the reference's edge order on one lane (deterministic).       */
+typedef enum umapgpu_sgd_mode {
+  UMAPGPU_SGD_HOGWILD = 0,
+  UMAPGPU_SGD_SERIAL = 1,
+  UMAPGPU_SGD_HOGWILD_RACY = 2
+} umapgpu_sgd_mode;
+
+/* UmapConfig (config.rs:206-235) with ManifoldParams (:9-52), GraphParams
+ * (:59-127) and OptimizationParams (:134-180).  Each Option<T> is a has_
+ * flag plus a value.  umapgpu_config_default() gives the reference defaults. */
+typedef struct umapgpu_config {
+  uint64_t n_components;
+  /* manifold */
+  float min_dist;
+  float spread;
+  int32_t has_a;
+  float a;
+  int32_t has_b;
+  float b;
+  /* graph */
+  uint64_t n_neighbors;
+  float local_connectivity;
+  float set_op_mix_ratio;
+  int32_t has_disconnection_distance;
+  float disconnection_distance;
+  int32_t symmetrize;
+  /* optimization */
+  int32_t has_n_epochs;
+  uint64_t n_epochs;
+  float learning_rate;
+  uint64_t negative_sample_rate;
+  float repulsion_strength;
+} umapgpu_config;
+
+typedef struct umapgpu_manifold umapgpu_manifold;   /* LearnedManifold (manifold.rs:15-38) */
+typedef struct umapgpu_optimizer umapgpu_optimizer; /* Optimizer (optimizer.rs:28-56) */
+typedef struct umapgpu_fitted umapgpu_fitted;       /* FittedUmap (embedding.rs:439-443) */
+
+/* Generic output metric callback: Metric::distance (metric.rs:36).  Writes
+ * d distance / d a into grad[0..dim) and returns the distance. */
+typedef float (*umapgpu_distance_fn)(const float* a, const float* b, uint64_t dim, float* grad, void* user);
+
+/* ---- library ----------------------------------------------------------- */
+UMAPGPU_API const char* umapgpu_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+UMAPGPU_API const char* umapgpu_last_error(void);
+/* Select the HIP device used by subsequent calls on this thread. */
+UMAPGPU_API umapgpu_status umapgpu_set_device(int32_t device);
+UMAPGPU_API umapgpu_status umapgpu_device_count(int32_t* count);
+
+/* ---- configuration ----------------------------------------------------- */
+/* UmapConfig::default() (config.rs:222-233, :45-51, :114-126, :170-179). */
+UMAPGPU_API void umapgpu_config_default(umapgpu_config* cfg);
+/* find_ab_params (umap/find_ab_params.rs:9-82): host f32 curve fit. */
+UMAPGPU_API umapgpu_status umapgpu_find_ab_params(float spread, float min_dist, float* a, float* b);
+
+/* ---- Umap::learn_manifold (embedding.rs:149-240) ------------------------ */
+/* n_samples = data.shape()[0] (the data values are never read,
+ * embedding.rs:155).  knn_indices is idx_rows x idx_cols and knn_dists is
+ * dist_rows x dist_cols, each row-major with the given row stride in
+ * elements.  metric_disconnection_threshold is the input metric's
+ * Metric::disconnection_threshold() (metric.rs:45-47; +inf for Euclidean),
+ * used when the config has no disconnection_distance (embedding.rs:169-173). */
+UMAPGPU_API umapgpu_status umapgpu_learn_manifold(const umapgpu_config* cfg, uint64_t n_samples,
+                                                  const uint32_t* knn_indices, uint64_t idx_rows, uint64_t idx_cols,
+                                                  uint64_t idx_stride, const float* knn_dists, uint64_t dist_rows,
+                                                  uint64_t dist_cols, uint64_t dist_stride,
+                                                  float metric_disconnection_threshold, umapgpu_manifold** out);
+/* Same, with knn arrays already resident on the device (contiguous n x k). */
+UMAPGPU_API umapgpu_status umapgpu_learn_manifold_device(const umapgpu_config* cfg, uint64_t n_samples,
+                                                         const uint32_t* knn_indices_device,
+                                                         const float* knn_dists_device, uint64_t n_neighbors,
+                                                         float metric_disconnection_threshold, umapgpu_manifold** out);
+
+/* LearnedManifold::{n_vertices, curve_params, graph().nnz()} (manifold.rs:41-54) */
+UMAPGPU_API umapgpu_status umapgpu_manifold_info(const umapgpu_manifold* m, uint64_t* n_vertices, uint64_t* nnz,
+                                                 float* a, float* b);
+/* LearnedManifold::graph() as CSR: indptr[n+1] (usize), indices[nnz] (u32),
+ * data[nnz] (f32); any pointer may be NULL to skip it. */
+UMAPGPU_API umapgpu_status umapgpu_manifold_copy_graph(const umapgpu_manifold* m, uint64_t* indptr,
+                                                       uint32_t* indices, float* data);
+/* LearnedManifold::{sigmas, rhos} (manifold.rs:22-29) */
+UMAPGPU_API umapgpu_status umapgpu_manifold_copy_sigmas_rhos(const umapgpu_manifold* m, float* sigmas, float* rhos);
+/* Counts logged by learn_manifold: edges removed by the disconnection
+ * distance (embedding.rs:186), vertices whose weights sum to 0 (:216-222). */
+UMAPGPU_API umapgpu_status umapgpu_manifold_disconnections(const umapgpu_manifold* m, uint64_t* edges_removed,
+                                                           uint64_t* vertices_disconnected);
+/* Per-stage device times of the last learn_manifold as JSON text (the
+ * reference's tracing::info! duration_ms events). Returns bytes needed. */
+UMAPGPU_API uint64_t umapgpu_manifold_timings_json(const umapgpu_manifold* m, char* buf, uint64_t len);
+/* Rebuild a manifold from host arrays (serde Deserialize, manifold.rs:14). */
+UMAPGPU_API umapgpu_status umapgpu_manifold_from_host(uint64_t n_vertices, uint64_t nnz, const uint64_t* indptr,
+                                                      const uint32_t* indices, const float* data,
+                                                      const float* sigmas, const float* rhos, float a, float b,
+                                                      umapgpu_manifold** out);
+/* #[derive(Clone)] (manifold.rs:14) */
+UMAPGPU_API umapgpu_status umapgpu_manifold_clone(const umapgpu_manifold* m, umapgpu_manifold** out);
+UMAPGPU_API void umapgpu_manifold_destroy(umapgpu_manifold* m);
+
+/* ---- Optimizer (optimizer.rs) ------------------------------------------ */
+/* Optimizer::new (optimizer.rs:74-290).  Consumes `manifold`.  init is
+ * n x dim row-major host memory (the owned Array2<f32>).  seed keys the
+ * counter RNG of negative sampling. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_new(umapgpu_manifold* manifold, const float* init, uint64_t n,
+                                                 uint64_t dim, uint64_t total_epochs, const umapgpu_config* cfg,
+                                                 int32_t metric_type, uint64_t seed, umapgpu_optimizer** out);
+/* Same with init resident on the device (n x dim contiguous). */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_new_device(umapgpu_manifold* manifold, const float* init_device,
+                                                        uint64_t n, uint64_t dim, uint64_t total_epochs,
+                                                        const umapgpu_config* cfg, int32_t metric_type, uint64_t seed,
+                                                        umapgpu_optimizer** out);
+/* Optimizer::step_epochs (optimizer.rs:297-366) for MetricType::Euclidean.
+ * UMAPGPU_OVERSTEP with the reference's message if n would pass total. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs(umapgpu_optimizer* opt, uint64_t n);
+/* Optimizer::step_epochs for MetricType::Generic: the reference's serial
+ * loop (layout/optimize_layout_generic.rs:35-124) on the host, calling the
+ * caller's Metric::distance for every sample. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* opt, uint64_t n,
+                                                                 umapgpu_distance_fn distance, void* user);
+UMAPGPU_API umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* opt, int32_t mode);
+/* current_epoch / total_epochs / remaining_epochs (optimizer.rs:369-381) */
+UMAPGPU_API uint64_t umapgpu_optimizer_current_epoch(const umapgpu_optimizer* opt);
+UMAPGPU_API uint64_t umapgpu_optimizer_total_epochs(const umapgpu_optimizer* opt);
+UMAPGPU_API uint64_t umapgpu_optimizer_remaining_epochs(const umapgpu_optimizer* opt);
+/* Optimizer::embedding() (optimizer.rs:384-386): copy n x dim to host. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_copy_embedding(const umapgpu_optimizer* opt, float* out);
+/* Device pointer of the embedding (valid until the next call on opt). */
+UMAPGPU_API const float* umapgpu_optimizer_embedding_device(const umapgpu_optimizer* opt);
+/* Optimizer::manifold() (optimizer.rs:389-391): borrowed handle. */
+UMAPGPU_API const umapgpu_manifold* umapgpu_optimizer_manifold(const umapgpu_optimizer* opt);
+/* Sizes of the optimizer state: edges kept by the threshold filter
+ * (optimizer.rs:110-131), points and embedding dimension. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_info(const umapgpu_optimizer* opt, uint64_t* n_edges,
+                                                  uint64_t* n_vertices, uint64_t* dim);
+/* Totals since creation: positive edge-updates and negative samples drawn;
+ * device time (ms) of the last step_epochs call. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_stats(const umapgpu_optimizer* opt, uint64_t* positive_updates,
+                                                   uint64_t* negative_samples, double* last_step_ms);
+/* Checkpoint (serde of Optimizer, optimizer.rs:27-56): host copies of every
+ * state array at a step_epochs boundary.  Arrays are n_edges long
+ * (head/tail u32, schedule f64) and embedding n x dim.  Any may be NULL. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_export_state(const umapgpu_optimizer* opt, uint32_t* head,
+                                                          uint32_t* tail, double* epochs_per_sample,
+                                                          double* epoch_of_next_sample,
+                                                          double* epochs_per_negative_sample,
+                                                          double* epoch_of_next_negative_sample, float* embedding);
+UMAPGPU_API umapgpu_status umapgpu_optimizer_export_params(const umapgpu_optimizer* opt, uint64_t* current_epoch,
+                                                           uint64_t* total_epochs, float* gamma, float* initial_alpha,
+                                                           double* negative_sample_rate, int32_t* metric_type,
+                                                           uint64_t* seed);
+/* Rebuild an optimizer from exported state; consumes `manifold`.  head must
+ * be non-decreasing (the reference's CSR row order). */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_import_state(
+    umapgpu_manifold* manifold, uint64_t n_edges, const uint32_t* head, const uint32_t* tail,
+    const double* epochs_per_sample, const double* epoch_of_next_sample, const double* epochs_per_negative_sample,
+    const double* epoch_of_next_negative_sample, const float* embedding, uint64_t dim, uint64_t current_epoch,
+    uint64_t total_epochs, float gamma, float initial_alpha, double negative_sample_rate, int32_t metric_type,
+    uint64_t seed, umapgpu_optimizer** out);
+/* Optimizer::into_fitted (optimizer.rs:397-403).  Consumes opt. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_into_fitted(umapgpu_optimizer* opt, const umapgpu_config* cfg,
+                                                         umapgpu_fitted** out);
+UMAPGPU_API void umapgpu_optimizer_destroy(umapgpu_optimizer* opt);
+
+/* ---- Umap::fit and FittedUmap (embedding.rs:274-337, :439-493) ---------- */
+/* Umap::fit: learn_manifold + Optimizer::new + step_epochs(total) +
+ * into_fitted + NaN rows for vertices whose weights sum to 0.  The output
+ * metric's type selects the SGD path (embedding.rs:311); Generic needs a
+ * distance callback. */
+UMAPGPU_API umapgpu_status umapgpu_fit(const umapgpu_config* cfg, uint64_t n_samples, const uint32_t* knn_indices,
+                                       uint64_t idx_rows, uint64_t idx_cols, uint64_t idx_stride,
+                                       const float* knn_dists, uint64_t dist_rows, uint64_t dist_cols,
+                                       uint64_t dist_stride, const float* init, uint64_t init_rows,
+                                       uint64_t init_cols, float metric_disconnection_threshold,
+                                       int32_t output_metric_type, umapgpu_distance_fn output_distance,
+                                       void* output_user, uint64_t seed, umapgpu_fitted** out);
+UMAPGPU_API umapgpu_status umapgpu_fitted_info(const umapgpu_fitted* f, uint64_t* n, uint64_t* dim);
+/* FittedUmap::embedding() (embedding.rs:462-464) */
+UMAPGPU_API umapgpu_status umapgpu_fitted_copy_embedding(const umapgpu_fitted* f, float* out);
+/* FittedUmap::manifold() (embedding.rs:486-488): borrowed handle. */
+UMAPGPU_API const umapgpu_manifold* umapgpu_fitted_manifold(const umapgpu_fitted* f);
+/* FittedUmap::config() (embedding.rs:491-493) */
+UMAPGPU_API umapgpu_status umapgpu_fitted_config(const umapgpu_fitted* f, umapgpu_config* cfg);
+/* Rebuild from host data (serde Deserialize); consumes `manifold`. */
+UMAPGPU_API umapgpu_status umapgpu_fitted_from_host(umapgpu_manifold* manifold, const float* embedding, uint64_t n,
+                                                    uint64_t dim, const umapgpu_config* cfg, umapgpu_fitted** out);
+UMAPGPU_API void umapgpu_fitted_destroy(umapgpu_fitted* f);
+
+/* ---- device memory and harness utilities (no reference counterpart) ----
+ * Used by benchmarks and tests to keep inputs resident in HBM. */
+UMAPGPU_API umapgpu_status umapgpu_device_alloc(uint64_t bytes, void** ptr);
+UMAPGPU_API umapgpu_status umapgpu_device_free(void* ptr);
+UMAPGPU_API umapgpu_status umapgpu_memcpy_htod(void* dst_device, const void* src_host, uint64_t bytes);
+UMAPGPU_API umapgpu_status umapgpu_memcpy_dtoh(void* dst_host, const void* src_device, uint64_t bytes);
+UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
+/* Exact brute-force Euclidean kNN (the caller-side producer the reference
+ * leaves to the user, README.md:37-43): row i of the output holds i itself
+ * at slot 0 (distance 0), then its k-1 nearest other points by ascending
+ * distance (ties by index).  data is n x dim row-major, all device
+ * pointers. */
+UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
+                                                         uint64_t k, uint32_t* indices_device,
+                                                         float* dists_device);
+/* Calibration hook: a plain streaming copy of `bytes` (a multiple of 16),
+ * used to scale hardware traffic counters into bytes. */
+UMAPGPU_API umapgpu_status umapgpu_debug_copy_device(const void* src_device, void* dst_device, uint64_t bytes);
+/* Test hook: y[i] = expf(x[i]) through the device's glibc-equivalent expf. */
+UMAPGPU_API umapgpu_status umapgpu_debug_expf_device(const float* x_device, float* y_device, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UMAPGPU_H */

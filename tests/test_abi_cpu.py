@@ -1,0 +1,115 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+header declares, and the host-only calls (defaults, find_ab_params, the
+reference's validation panics) behave like the reference without a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import ring_test_data
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(REPO, "include", "umapgpu.h")).read()
+    return sorted(set(re.findall(r"UMAPGPU_API[^;]*?\b(umapgpu_[a-z0-9_]+)\s*\(", txt, re.S)))
+
+
+def test_exports_every_declared_symbol(U):
+    names = declared_symbols()
+    assert len(names) >= 45
+    L = U.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing
+
+
+def test_config_defaults_match_reference(U):
+    c = U.UmapConfig()._to_c()
+    d = U._Config()
+    U.lib().umapgpu_config_default(d)
+    # config.rs:45-51, :114-126, :170-179, :222-233
+    assert (d.n_components, d.min_dist, d.spread, d.has_a, d.has_b) == (2, np.float32(0.1), 1.0, 0, 0)
+    assert (d.n_neighbors, d.local_connectivity, d.set_op_mix_ratio, d.has_disconnection_distance,
+            d.symmetrize) == (15, 1.0, 1.0, 0, 1)
+    assert (d.has_n_epochs, d.learning_rate, d.negative_sample_rate, d.repulsion_strength) == (0, 1.0, 5, 1.0)
+    for f, _ in U._Config._fields_:
+        assert getattr(c, f) == getattr(d, f)
+
+
+@pytest.mark.parametrize("spread,min_dist", [(1.0, 0.1), (1.0, 0.05), (1.0, 0.0), (1.0, 0.5), (2.0, 0.3)])
+def test_find_ab_params_bit_exact_with_oracle(U, oracle, spread, min_dist):
+    a, b = U.find_ab_params(spread, min_dist)
+    ao, bo, _ = oracle.find_ab_params(spread, min_dist)
+    assert a == ao and b == bo
+
+
+def _learn(U, cfg, n=50, k=10, cols=None, rows=None, data_rows=None):
+    data, idx, d, _ = ring_test_data(seed=0, n=n, k=k)
+    if cols is not None:
+        d = d[:, :cols]
+    if rows is not None:
+        d = d[:rows]
+    return U.Umap(cfg).learn_manifold(np.zeros((data_rows or n, 3)), idx, d)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda c: setattr(c.graph, "set_op_mix_ratio", 1.5), "set_op_mix_ratio must be between 0.0 and 1.0, got 1.5"),
+    (lambda c: setattr(c.graph, "set_op_mix_ratio", -0.25), "set_op_mix_ratio must be between 0.0 and 1.0, got -0.25"),
+    (lambda c: setattr(c.graph, "n_neighbors", 1), "n_neighbors must be >= 2, got 1"),
+    (lambda c: setattr(c.optimization, "repulsion_strength", -1.0), "repulsion_strength cannot be negative, got -1"),
+    (lambda c: setattr(c.manifold, "min_dist", 2.0), "min_dist (2) must be <= spread (1)"),
+    (lambda c: (setattr(c.manifold, "min_dist", -0.5), setattr(c.manifold, "spread", 1.0)),
+     "min_dist cannot be negative, got -0.5"),
+    (lambda c: setattr(c.optimization, "learning_rate", -0.1), "learning_rate must be positive, got -0.1"),
+    (lambda c: setattr(c, "n_components", 0), "n_components must be >= 1, got 0"),
+])
+def test_validation_panics(U, mutate, msg):
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = 10
+    mutate(cfg)
+    with pytest.raises(U.UmapPanic) as e:
+        _learn(U, cfg)
+    assert str(e.value) == msg
+
+
+def test_shape_panics(U):
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = 10
+    with pytest.raises(U.UmapPanic, match=re.escape("knn_dists and knn_indices must have the same shape, "
+                                                    "got [50, 9] vs [50, 10]")):
+        _learn(U, cfg, cols=9)
+    cfg.graph.n_neighbors = 9
+    data, idx, d, _ = ring_test_data(seed=0)
+    with pytest.raises(U.UmapPanic, match="knn_dists has 10 neighbors but n_neighbors is 9"):
+        U.Umap(cfg).learn_manifold(data, idx, d)
+    cfg.graph.n_neighbors = 10
+    with pytest.raises(U.UmapPanic, match="knn_dists has 50 samples but data has 60 samples"):
+        _learn(U, cfg, data_rows=60)
+    data, idx, d, _ = ring_test_data(seed=0, n=10, k=10)
+    with pytest.raises(U.UmapPanic, match=re.escape("Number of samples (10) must be > n_neighbors (10)")):
+        U.Umap(cfg).learn_manifold(data, idx, d)
+
+
+def test_fit_init_panics(U):
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = 10
+    data, idx, d, init = ring_test_data(seed=0)
+    with pytest.raises(U.UmapPanic, match="init has 3 components but n_components is 2"):
+        U.Umap(cfg).fit(data, idx, d, np.zeros((50, 3), np.float32))
+    with pytest.raises(U.UmapPanic, match="init has 49 samples but data has 50 samples"):
+        U.Umap(cfg).fit(data, idx, d, init[:49])
+
+
+def test_rust_float_display(U):
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = 10
+    for v, text in [(1e-7, "0.0000001"), (1e20, "100000000000000000000"), (2.5, "2.5"), (float("nan"), None)]:
+        cfg.optimization.learning_rate = -v if v == v else v
+        if v != v:
+            _ = text  # NaN passes the `< 0.0` check, as in the reference
+            continue
+        with pytest.raises(U.UmapPanic) as e:
+            _learn(U, cfg)
+        assert str(e.value) == "learning_rate must be positive, got -" + text

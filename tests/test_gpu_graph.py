@@ -1,0 +1,169 @@
+"""Graph stage on the GPU (Umap::learn_manifold) against the CPU restatement.
+
+Bar: sigma, rho, CSR row pointers, column indices and weights bit-identical to
+the restatement (which follows the reference's f32 operation order and glibc
+expf); the tolerance the task allows (sigma/rho 1e-5, weights 1e-6 relative) is
+asserted too, so a failure says which bar broke.
+"""
+import numpy as np
+import pytest
+
+from helpers import gmm, knn_exact, ring_test_data
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _cfg(U, k, **kw):
+    c = U.UmapConfig()
+    c.graph.n_neighbors = k
+    for key, v in kw.items():
+        for grp in (c.graph, c.manifold, c.optimization):
+            if hasattr(grp, key):
+                setattr(grp, key, v)
+                break
+        else:
+            setattr(c, key, v)
+    return c
+
+
+def _compare(U, oracle, idx, d, n_samples=None, **kw):
+    n, k = d.shape
+    cfg = _cfg(U, k, **kw)
+    m = U.Umap(cfg).learn_manifold(np.zeros((n_samples or n, 1)), idx, d)
+    g = m.graph()
+    s, r = m.sigmas_rhos()
+    disc = kw.get("disconnection_distance", None)
+    om = oracle.learn_manifold(idx, d, n_neighbors=k, local_connectivity=cfg.graph.local_connectivity,
+                               set_op_mix_ratio=cfg.graph.set_op_mix_ratio, symmetrize=cfg.graph.symmetrize,
+                               disconnection_distance=np.inf if disc is None else disc,
+                               min_dist=cfg.manifold.min_dist, spread=cfg.manifold.spread)
+    print(f"[graph parity] n={n} k={k} {kw} nnz gpu={g.nnz()} oracle={om.nnz}", flush=True)
+    # tolerance bars first (informative), then bit-exactness
+    assert np.allclose(s, om.sigmas, rtol=1e-5, atol=0) and np.allclose(r, om.rhos, rtol=1e-5, atol=0)
+    assert np.array_equal(g.indptr(), om.indptr)
+    assert np.array_equal(g.indices(), om.indices)
+    assert np.allclose(g.data(), om.data, rtol=1e-6, atol=0)
+    assert np.array_equal(s.view(np.uint32), om.sigmas.view(np.uint32))
+    assert np.array_equal(r.view(np.uint32), om.rhos.view(np.uint32))
+    assert np.array_equal(g.data().view(np.uint32), om.data.view(np.uint32))
+    assert m.curve_params() == (om.a, om.b)
+    er, vd = m.disconnections()
+    assert (er, vd) == (om.edges_removed, om.vertices_disconnected)
+    return m, om
+
+
+def test_device_expf_matches_glibc(gpu):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-110, 0, 4_000_000), rng.uniform(-104.5, -103.0, 1_000_000),
+                        rng.uniform(0, 90, 500_000)]).astype(F)
+    x = np.concatenate([x, np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 88.72284, -103.97208], F)])
+    y = gpu.debug_expf(x)
+    ref = np.exp(x.astype(np.float64))  # glibc expf via numpy float32 would also do; compare with libc below
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.expf.restype = ctypes.c_float
+    libm.expf.argtypes = [ctypes.c_float]
+    sample = np.concatenate([np.arange(0, x.size, 97), np.arange(x.size - 7, x.size)])
+    g = np.array([libm.expf(float(v)) for v in x[sample]], F)
+    assert np.array_equal(y[sample].view(np.uint32)[~np.isnan(g)], g.view(np.uint32)[~np.isnan(g)])
+    assert np.allclose(y[np.isfinite(ref)], ref[np.isfinite(ref)], rtol=1e-6, atol=1e-44)
+
+
+def test_ring_fixture(gpu, oracle):
+    _, idx, d, _ = ring_test_data(seed=0)
+    _compare(gpu, oracle, idx, d)
+
+
+@pytest.mark.parametrize("n,dim,k,seed", [(2000, 16, 15, 1), (3000, 32, 30, 2), (700, 8, 2, 3), (500, 8, 3, 4),
+                                         (1200, 16, 64, 5), (300, 4, 100, 6)])
+def test_gmm_exact_knn(gpu, oracle, n, dim, k, seed):
+    x, _ = gmm(n, dim, 7, seed=seed)
+    idx, d = knn_exact(x, k)
+    _compare(gpu, oracle, idx, d)
+
+
+@pytest.mark.parametrize("lc", [0.0, 0.5, 1.0, 1.5, 3.0])
+def test_local_connectivity(gpu, oracle, lc):
+    x, _ = gmm(900, 12, 5, seed=7)
+    idx, d = knn_exact(x, 15)
+    _compare(gpu, oracle, idx, d, local_connectivity=lc)
+
+
+@pytest.mark.parametrize("mix", [0.0, 0.25, 0.5, 0.75, 1.0])
+def test_set_op_mix_ratio(gpu, oracle, mix):
+    x, _ = gmm(1000, 12, 5, seed=8)
+    idx, d = knn_exact(x, 12)
+    _compare(gpu, oracle, idx, d, set_op_mix_ratio=mix)
+
+
+def test_no_symmetrize(gpu, oracle):
+    x, _ = gmm(1500, 12, 5, seed=9)
+    idx, d = knn_exact(x, 15)
+    _compare(gpu, oracle, idx, d, symmetrize=False)
+
+
+def test_disconnection_distance_and_sentinels(gpu, oracle):
+    x, _ = gmm(800, 12, 5, seed=10)
+    idx, d = knn_exact(x, 15)
+    idx[3, 5:] = 0xFFFFFFFF      # kNN ran short: sentinel slots are skipped
+    idx[10, 2] = 10              # explicit self loop in a later slot
+    thr = float(np.quantile(d[:, 1:], 0.9))
+    _, om = _compare(gpu, oracle, idx, d, disconnection_distance=thr)
+    assert om.edges_removed > 0
+    d2 = d.copy()
+    d2[20, :] = np.inf           # a fully disconnected row (+inf >= +inf)
+    _, om = _compare(gpu, oracle, idx, d2)
+    assert om.vertices_disconnected >= 1
+
+
+def test_zero_distance_rows_use_global_mean(gpu, oracle):
+    x, _ = gmm(600, 8, 4, seed=11)
+    idx, d = knn_exact(x, 10)
+    d[[5, 77, 300], :] = 0.0  # duplicates: rho = 0 -> floor from the mean of all distances
+    m, _ = _compare(gpu, oracle, idx, d)
+    assert m.timings()["used_global_mean"] is True
+
+
+def test_strided_input(gpu, oracle):
+    x, _ = gmm(700, 8, 4, seed=12)
+    idx, d = knn_exact(x, 20)
+    wide_i = np.zeros((700, 25), np.uint32)
+    wide_d = np.zeros((700, 25), F)
+    wide_i[:, :20], wide_d[:, :20] = idx, d
+    cfg = _cfg(gpu, 20)
+    m = gpu.Umap(cfg).learn_manifold(np.zeros((700, 1)), wide_i[:, :20], wide_d[:, :20])
+    om = oracle.learn_manifold(idx, d)
+    g = m.graph()
+    assert np.array_equal(g.indptr(), om.indptr) and np.array_equal(g.indices(), om.indices)
+    assert np.array_equal(g.data().view(np.uint32), om.data.view(np.uint32))
+
+
+def test_unsorted_ring_distances_large(gpu, oracle):
+    """The reference's own test fixture shape (unsorted random distances) at a
+    larger n, where rows are not sorted by distance and rho is not slot 1."""
+    _, idx, d, _ = ring_test_data(seed=3, n=5000, k=20)
+    _compare(gpu, oracle, idx, d)
+
+
+def test_properties_at_scale(gpu):
+    """200k-point graph: properties that hold at any size (the oracle is too
+    slow here): symmetric structure and values for mix=1, sorted rows, weights
+    in (0, 1]."""
+    n, k = 200_000, 15
+    x, _ = gmm(n, 8, 50, seed=13)
+    dx = gpu.DeviceArray.from_host(x)
+    di, dd = gpu.knn_bruteforce_device(dx, k)
+    cfg = _cfg(gpu, k)
+    m = gpu.learn_manifold_device(cfg, di, dd)
+    g = m.graph()
+    ip, ix, dv = g.indptr().astype(np.int64), g.indices().astype(np.int64), g.data()
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    assert np.all((dv > 0) & (dv <= 1))
+    same_row = rows[1:] == rows[:-1]
+    assert np.all(ix[1:][same_row] > ix[:-1][same_row])
+    key = rows * n + ix
+    tkey = ix * n + rows
+    assert np.array_equal(key, np.sort(tkey))             # structure symmetric (key is sorted: CSR order)
+    pos = np.searchsorted(key, tkey)
+    assert np.array_equal(dv[pos].view(np.uint32), dv.view(np.uint32))  # values symmetric (mix=1)

@@ -1,0 +1,154 @@
+"""Optimizer::new and step_epochs on the GPU against the CPU restatement.
+
+* Optimizer::new (filter, head/tail, f64 schedule, [0,10] normalisation):
+  bit-identical state.
+* SERIAL mode (the reference's edge order on one lane) vs the sequential CPU
+  run: embedding within 1e-5 after one epoch (and after several), schedule
+  bit-identical.  Tolerance 1e-5 absolute on coordinates in [0, 10].
+* HOGWILD modes: the f64 schedule recurrence is RNG-free, so the counts of
+  positive updates and negative samples and the schedule arrays equal the
+  sequential run exactly; the embedding is compared statistically (fuzzy
+  cross-entropy within 5% of the CPU Hogwild run).
+"""
+import numpy as np
+import pytest
+
+from helpers import gmm, knn_exact, ring_test_data, uniform_init
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _setup(U, oracle, n=1500, dim=16, k=15, seed=0, total=50, n_components=2, **cfgkw):
+    x, _ = gmm(n, dim, 6, seed=seed)
+    idx, d = knn_exact(x, k)
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = k
+    cfg.n_components = n_components
+    for key, v in cfgkw.items():
+        setattr(cfg.optimization, key, v)
+    init = uniform_init(n, n_components, seed=seed + 100)
+    m = U.Umap(cfg).learn_manifold(x, idx, d)
+    om = oracle.learn_manifold(idx, d)
+    opt = U.Optimizer.new(m, init, total, cfg, U.MetricType.Euclidean, seed=seed)
+    ost = oracle.OptimizerState(om, init, total, negative_sample_rate=cfg.optimization.negative_sample_rate,
+                                learning_rate=cfg.optimization.learning_rate,
+                                repulsion_strength=cfg.optimization.repulsion_strength, seed=seed)
+    print(f"[optimizer] n={n} k={k} total={total} dim={n_components} edges={ost.n_edges}", flush=True)
+    return x, idx, cfg, opt, ost
+
+
+def _assert_state_equal(st, ost):
+    for k, ok in (("head", "head"), ("tail", "tail"), ("epochs_per_sample", "eps"),
+                  ("epoch_of_next_sample", "eons"), ("epochs_per_negative_sample", "epns"),
+                  ("epoch_of_next_negative_sample", "eonns")):
+        assert np.array_equal(st[k], getattr(ost, ok)), k
+
+
+@pytest.mark.parametrize("total", [50, 5, 500])
+def test_optimizer_new_bit_identical(gpu, oracle, total):
+    _, _, _, opt, ost = _setup(gpu, oracle, total=total)
+    st = opt.export_state()
+    _assert_state_equal(st, ost)
+    assert np.array_equal(st["embedding"].view(np.uint32), ost.embedding.view(np.uint32))
+    assert (opt.current_epoch(), opt.total_epochs(), opt.remaining_epochs()) == (0, total, total)
+
+
+@pytest.mark.parametrize("n_components", [2, 3, 1, 9])
+def test_serial_mode_matches_sequential_cpu(gpu, oracle, n_components):
+    _, _, _, opt, ost = _setup(gpu, oracle, n=800, total=30, n_components=n_components)
+    opt.set_mode(gpu.SGD_SERIAL)
+    for chunk in (2, 1, 5):  # epoch 0 is a no-op; compare after each call
+        opt.step_epochs(chunk)
+        ost.step_epochs(chunk, "serial")
+        y = opt.embedding()
+        err = np.abs(y - ost.embedding).max()
+        print(f"[serial] dim={n_components} epoch={opt.current_epoch()} max|gpu-cpu|={err:.3g}", flush=True)
+        assert err <= 1e-5
+        _assert_state_equal(opt.export_state(), ost)
+    p, q, _ = opt.stats()
+    assert (p, q) == tuple(int(c) for c in ost.counts)
+
+
+@pytest.mark.parametrize("mode", ["hogwild", "hogwild_racy"])
+def test_hogwild_schedule_exact_and_quality(gpu, oracle, mode):
+    import umap_eval as E
+    total = 200
+    x, idx, cfg, opt, ost = _setup(gpu, oracle, n=3000, total=total, seed=3)
+    opt.set_mode(gpu.SGD_HOGWILD if mode == "hogwild" else gpu.SGD_HOGWILD_RACY)
+    y0 = opt.embedding()
+    opt.step_epochs(total)
+    ost.step_epochs(total, "hogwild")
+    p, q, ms = opt.stats()
+    assert (p, q) == tuple(int(c) for c in ost.counts)  # RNG-free schedule
+    st = opt.export_state()
+    for k, ok in (("epoch_of_next_sample", "eons"), ("epoch_of_next_negative_sample", "eonns")):
+        assert np.array_equal(st[k], getattr(ost, ok))
+    m = opt.manifold()
+    g = m.graph()
+    a, b = m.curve_params()
+    ce_gpu = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), st["embedding"], a, b)
+    ce_cpu = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), ost.embedding, a, b)
+    init_ce = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), y0, a, b)
+    print(f"[{mode}] CE gpu={ce_gpu:.1f} cpu={ce_cpu:.1f} init={init_ce:.1f} updates={p} step_ms={ms:.2f}",
+          flush=True)
+    assert ce_gpu < 0.9 * init_ce
+    assert abs(ce_gpu - ce_cpu) / ce_cpu < 0.05
+
+
+def test_chunked_resume_equals_uninterrupted_serial(gpu, oracle):
+    """Checkpoint at a step_epochs boundary: export, rebuild, continue.  In
+    serial mode this equals the uninterrupted run with the same chunking."""
+    _, _, cfg, opt, _ = _setup(gpu, oracle, n=600, total=20, seed=5)
+    _, _, _, ref, _ = _setup(gpu, oracle, n=600, total=20, seed=5)
+    opt.set_mode(gpu.SGD_SERIAL)
+    ref.set_mode(gpu.SGD_SERIAL)
+    opt.step_epochs(8)
+    ref.step_epochs(8)
+    state = opt.to_state()
+    del opt
+    opt2 = gpu.Optimizer.from_state(state)
+    opt2.set_mode(gpu.SGD_SERIAL)
+    assert (opt2.current_epoch(), opt2.remaining_epochs()) == (8, 12)
+    opt2.step_epochs(12)
+    ref.step_epochs(12)
+    assert np.array_equal(opt2.embedding(), ref.embedding())
+
+
+def test_generic_metric_host_path_matches_restatement(gpu, oracle):
+    """MetricType::Generic runs the reference's serial generic loop with the
+    caller's Metric::distance; with the Euclidean metric as the callback it
+    equals the restatement of optimize_layout_generic.rs."""
+    _, idx, d, init = ring_test_data(seed=7, n=120, k=10)
+    cfg = gpu.UmapConfig()
+    cfg.graph.n_neighbors = 10
+    m = gpu.Umap(cfg).learn_manifold(np.zeros((120, 1)), idx, d)
+    opt = gpu.Optimizer.new(m, init, 6, cfg, gpu.MetricType.Generic, seed=2)
+    ost = oracle.OptimizerState(oracle.learn_manifold(idx, d), init, 6, seed=2)
+    metric = gpu.EuclideanMetric()
+    opt.step_epochs(3, metric)
+    ost.step_epochs_generic_euclid(3)
+    assert np.allclose(opt.embedding(), ost.embedding, atol=1e-5)
+    with pytest.raises(gpu.UmapPanic):
+        opt.step_epochs(1)  # Generic needs the metric
+
+
+def test_overstep_panics(gpu, oracle):
+    _, _, _, opt, _ = _setup(gpu, oracle, n=300, total=20)
+    opt.step_epochs(18)
+    with pytest.raises(gpu.UmapPanic, match=r"Cannot step 5 epochs: would exceed total_epochs 20 \(current: 18\)"):
+        opt.step_epochs(5)
+    assert opt.current_epoch() == 18
+
+
+@pytest.mark.parametrize("n_components", [2, 3, 9])
+def test_racy_and_atomic_modes_share_schedule(gpu, oracle, n_components):
+    _, _, _, a, ost = _setup(gpu, oracle, n=1000, total=40, seed=9, n_components=n_components)
+    _, _, _, b, _ = _setup(gpu, oracle, n=1000, total=40, seed=9, n_components=n_components)
+    a.set_mode(gpu.SGD_HOGWILD)
+    b.set_mode(gpu.SGD_HOGWILD_RACY)
+    a.step_epochs(40)
+    b.step_epochs(40)
+    ost.step_epochs(40, "serial")
+    assert a.stats()[:2] == b.stats()[:2] == tuple(int(c) for c in ost.counts)
+    assert np.all(np.isfinite(a.embedding())) and np.all(np.isfinite(b.embedding()))

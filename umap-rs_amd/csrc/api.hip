@@ -1,0 +1,886 @@
+// C ABI of libumapgpu (include/umapgpu.h): handles, validation with the
+// reference's panic texts, and orchestration of the device stages.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <charconv>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/umapgpu.h"
+#include "graph.hpp"
+#include "knn.hpp"
+#include "optimize.hpp"
+#include "primitives.hpp"
+
+using namespace umapgpu;
+
+// ---------------------------------------------------------------------------
+// errors, buffers, streams
+
+namespace umapgpu {
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  std::string msg = std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") in " + what +
+                    " at " + file + ":" + std::to_string(line);
+  if (e == hipErrorOutOfMemory) throw Error(Err::OutOfMemory, msg);
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) throw Error(Err::NoDevice, msg);
+  throw Error(Err::Hip, msg);
+}
+
+void DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    throw_hip(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  bytes = n;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+}  // namespace umapgpu
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local int g_device = -1;
+
+hipStream_t stream() {
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  UMAP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error(Err::NoDevice, "device index out of range");
+  if (!streams[dev]) UMAP_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+  return streams[dev];
+}
+
+void ensure_device() {
+  if (g_device >= 0) UMAP_HIP(hipSetDevice(g_device));
+}
+
+template <typename F>
+umapgpu_status guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return UMAPGPU_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    switch (e.code) {
+      case Err::InvalidArgument: return UMAPGPU_INVALID_ARGUMENT;
+      case Err::OutOfMemory: return UMAPGPU_OUT_OF_MEMORY;
+      case Err::Hip: return UMAPGPU_HIP_ERROR;
+      case Err::Overstep: return UMAPGPU_OVERSTEP;
+      case Err::NoDevice: return UMAPGPU_NO_DEVICE;
+      case Err::Unsupported: return UMAPGPU_UNSUPPORTED;
+    }
+    return UMAPGPU_HIP_ERROR;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return UMAPGPU_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return UMAPGPU_HIP_ERROR;
+  }
+}
+
+// Rust's `{}` (Display) for f32: shortest round-trip digits, never in
+// exponent form; NaN prints as "NaN".
+std::string rs_f32(float v) {
+  if (v != v) return "NaN";
+  if (isinf(v)) return v > 0 ? "inf" : "-inf";
+  // shortest round-trip digits (as Rust's formatter), then laid out without
+  // an exponent: 1e20 -> "100000000000000000000", 1e-7 -> "0.0000001"
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::scientific);
+  std::string sci(buf, r.ptr), out;
+  size_t pos = 0;
+  if (sci[0] == '-') {
+    out = "-";
+    pos = 1;
+  }
+  const size_t epos = sci.find('e');
+  std::string digits;
+  for (size_t i = pos; i < epos; ++i)
+    if (sci[i] != '.') digits += sci[i];
+  const int e = std::stoi(sci.substr(epos + 1));
+  const int nd = (int)digits.size();
+  if (e >= nd - 1) {
+    out += digits + std::string((size_t)(e - (nd - 1)), '0');
+  } else if (e >= 0) {
+    out += digits.substr(0, (size_t)e + 1) + "." + digits.substr((size_t)e + 1);
+  } else {
+    out += "0." + std::string((size_t)(-e - 1), '0') + digits;
+  }
+  return out;
+}
+std::string rs_u64(uint64_t v) { return std::to_string(v); }
+
+[[noreturn]] void panic(const std::string& m) { throw Error(Err::InvalidArgument, m); }
+
+// Umap::validate_parameters (embedding.rs:339-429), same checks and order.
+void validate_parameters(const umapgpu_config& c, uint64_t n_samples, uint64_t idx_rows, uint64_t idx_cols,
+                         uint64_t dist_rows, uint64_t dist_cols) {
+  if (c.set_op_mix_ratio < 0.0f || c.set_op_mix_ratio > 1.0f)
+    panic("set_op_mix_ratio must be between 0.0 and 1.0, got " + rs_f32(c.set_op_mix_ratio));
+  if (c.n_neighbors < 2) panic("n_neighbors must be >= 2, got " + rs_u64(c.n_neighbors));
+  if (c.repulsion_strength < 0.0f)
+    panic("repulsion_strength cannot be negative, got " + rs_f32(c.repulsion_strength));
+  if (c.min_dist > c.spread)
+    panic("min_dist (" + rs_f32(c.min_dist) + ") must be <= spread (" + rs_f32(c.spread) + ")");
+  if (c.min_dist < 0.0f) panic("min_dist cannot be negative, got " + rs_f32(c.min_dist));
+  if (c.learning_rate < 0.0f) panic("learning_rate must be positive, got " + rs_f32(c.learning_rate));
+  if (c.n_components < 1) panic("n_components must be >= 1, got " + rs_u64(c.n_components));
+  if (dist_rows != idx_rows || dist_cols != idx_cols)
+    panic("knn_dists and knn_indices must have the same shape, got [" + rs_u64(dist_rows) + ", " +
+          rs_u64(dist_cols) + "] vs [" + rs_u64(idx_rows) + ", " + rs_u64(idx_cols) + "]");
+  if (dist_cols != c.n_neighbors)
+    panic("knn_dists has " + rs_u64(dist_cols) + " neighbors but n_neighbors is " + rs_u64(c.n_neighbors));
+  if (dist_rows != n_samples)
+    panic("knn_dists has " + rs_u64(dist_rows) + " samples but data has " + rs_u64(n_samples) + " samples");
+  if (n_samples <= c.n_neighbors)
+    panic("Number of samples (" + rs_u64(n_samples) + ") must be > n_neighbors (" + rs_u64(c.n_neighbors) + ")");
+}
+
+// find_ab_params (umap/find_ab_params.rs:9-82): host f32 gradient descent
+// with glibc powf/logf/expf, built with -ffp-contract=off like the
+// reference's rustc output.
+void find_ab(float spread, float min_dist, float* a_out, float* b_out) {
+  enum { NP = 300 };
+  float xv[NP], yv[NP];
+  for (int i = 0; i < NP; i++) xv[i] = ((spread * 3.0f) * (float)i) / ((float)NP - 1.0f);
+  for (int i = 0; i < NP; i++) yv[i] = (xv[i] < min_dist) ? 1.0f : expf(-(xv[i] - min_dist) / spread);
+  float a = 1.5f, b = 0.9f;
+  const float lr = 0.01f;
+  for (int it = 0; it < 1000; it++) {
+    float ga = 0.0f, gb = 0.0f, te = 0.0f;
+    for (int i = 0; i < NP; i++) {
+      const float x = xv[i];
+      const float x2b = powf(x, 2.0f * b);
+      const float den = 1.0f + a * x2b;
+      const float err = 1.0f / den - yv[i];
+      te = te + err * err;
+      ga = ga + (2.0f * err) * (-x2b / (den * den));
+      if (x > 0.0f) gb = gb + (2.0f * err) * ((((-2.0f * a) * x2b) * logf(x)) / (den * den));
+    }
+    a = a - (lr * ga) / (float)NP;
+    b = b - (lr * gb) / (float)NP;
+    a = a < 0.001f ? 0.001f : (a > 10.0f ? 10.0f : a);
+    b = b < 0.001f ? 0.001f : (b > 10.0f ? 10.0f : b);
+    if (te / (float)NP < 1e-7f) break;
+  }
+  *a_out = a;
+  *b_out = b;
+}
+
+// raise_disconnected_warning (umap/raise_disconnected_warning.rs:4-26)
+std::string disconnected_warning(uint64_t edges_removed, uint64_t vertices_disconnected, float dist,
+                                 uint64_t total_rows, double threshold) {
+  const uint64_t lim = (uint64_t)(threshold * (double)total_rows);
+  const std::string d = rs_f32(dist);
+  if (vertices_disconnected == 0 && edges_removed > 0)
+    return "Disconnection_distance = " + d + " has removed " + rs_u64(edges_removed) +
+           " edges. This is not a problem as no vertices were disconnected.";
+  if (vertices_disconnected > 0 && vertices_disconnected <= lim)
+    return "A few of your vertices were disconnected from the manifold.  This shouldn't cause problems.\n"
+           "Disconnection_distance = " + d + " has removed " + rs_u64(edges_removed) +
+           " edges.\nIt has only fully disconnected " + rs_u64(vertices_disconnected) +
+           " vertices.\nUse umap.utils.disconnected_vertices() to identify them.";
+  if (vertices_disconnected > lim)
+    return "A large number of your vertices were disconnected from the manifold.\nDisconnection_distance = " + d +
+           " has removed " + rs_u64(edges_removed) + " edges.\nIt has fully disconnected " +
+           rs_u64(vertices_disconnected) +
+           " vertices.\nYou might consider using find_disconnected_points() to find and remove these points from "
+           "your data.\nUse umap.utils.disconnected_vertices() to identify them.";
+  return "";
+}
+
+__global__ void nan_rows_kernel(float* emb, const uint8_t* flags, uint64_t n, uint64_t dim) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n * dim && flags[i / dim]) emb[i] = __int_as_float(0x7fc00000);
+}
+
+}  // namespace
+
+struct umapgpu_manifold {
+  GraphResult g;
+  uint64_t n_vertices = 0;
+  float a = 0, b = 0;
+  std::string warning;
+};
+
+struct umapgpu_optimizer {
+  OptimizerState st;
+  std::unique_ptr<umapgpu_manifold> manifold;
+};
+
+struct umapgpu_fitted {
+  DevBuf embedding;
+  uint64_t n = 0, dim = 0;
+  std::unique_ptr<umapgpu_manifold> manifold;
+  umapgpu_config config;
+};
+
+namespace {
+
+void upload_strided(void* dst, const void* src, uint64_t rows, uint64_t cols, uint64_t stride, size_t elem,
+                    hipStream_t s) {
+  if (rows * cols == 0) return;
+  if (stride == cols) {
+    UMAP_HIP(hipMemcpyAsync(dst, src, rows * cols * elem, hipMemcpyHostToDevice, s));
+  } else {
+    UMAP_HIP(hipMemcpy2DAsync(dst, cols * elem, src, stride * elem, cols * elem, rows, hipMemcpyHostToDevice, s));
+  }
+}
+
+std::unique_ptr<umapgpu_manifold> learn(const umapgpu_config& c, uint64_t n_samples, const uint32_t* idx_dev,
+                                        const float* dist_dev, uint64_t k, bool contiguous, float metric_thr) {
+  auto m = std::make_unique<umapgpu_manifold>();
+  if (c.has_a && c.has_b) {  // embedding.rs:161-166
+    m->a = c.a;
+    m->b = c.b;
+  } else {
+    find_ab(c.spread, c.min_dist, &m->a, &m->b);
+  }
+  const float disc = c.has_disconnection_distance ? c.disconnection_distance : metric_thr;  // :169-173
+  GraphParams p;
+  p.n = n_samples;
+  p.k = (uint32_t)k;
+  p.target = log2f((float)c.n_neighbors) * 1.0f;  // smooth_knn_dist.rs:71, bandwidth 1
+  p.n_iter = 64;
+  p.local_connectivity = c.local_connectivity;
+  p.disconnection_distance = disc;
+  p.set_op_mix_ratio = c.set_op_mix_ratio;
+  p.symmetrize = c.symmetrize != 0;
+  p.contiguous_input = contiguous;
+  build_graph(idx_dev, dist_dev, p, &m->g, stream());
+  m->n_vertices = n_samples;
+  m->warning = disconnected_warning(m->g.edges_removed, m->g.vertices_disconnected, disc, n_samples, 0.1);
+  return m;
+}
+
+std::unique_ptr<umapgpu_manifold> clone_manifold(const umapgpu_manifold& src) {
+  hipStream_t s = stream();
+  auto m = std::make_unique<umapgpu_manifold>();
+  m->n_vertices = src.n_vertices;
+  m->a = src.a;
+  m->b = src.b;
+  m->warning = src.warning;
+  const GraphResult& a = src.g;
+  GraphResult& b = m->g;
+  b.edges_removed = a.edges_removed;
+  b.vertices_disconnected = a.vertices_disconnected;
+  b.nnz_directed = a.nnz_directed;
+  b.used_global_mean = a.used_global_mean;
+  b.timings = a.timings;
+  const uint64_t n = a.graph.n, nnz = a.graph.nnz;
+  b.graph.n = n;
+  b.graph.nnz = nnz;
+  b.graph.indptr = DevBuf((n + 1) * 8);
+  b.graph.indices = DevBuf(std::max<uint64_t>(nnz, 1) * 4);
+  b.graph.data = DevBuf(std::max<uint64_t>(nnz, 1) * 4);
+  b.sigmas = DevBuf(std::max<uint64_t>(n, 1) * 4);
+  b.rhos = DevBuf(std::max<uint64_t>(n, 1) * 4);
+  b.zero_rows = DevBuf(std::max<uint64_t>(n, 1));
+  UMAP_HIP(hipMemcpyAsync(b.graph.indptr.p, a.graph.indptr.p, (n + 1) * 8, hipMemcpyDeviceToDevice, s));
+  if (nnz) {
+    UMAP_HIP(hipMemcpyAsync(b.graph.indices.p, a.graph.indices.p, nnz * 4, hipMemcpyDeviceToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(b.graph.data.p, a.graph.data.p, nnz * 4, hipMemcpyDeviceToDevice, s));
+  }
+  if (n) {
+    UMAP_HIP(hipMemcpyAsync(b.sigmas.p, a.sigmas.p, n * 4, hipMemcpyDeviceToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(b.rhos.p, a.rhos.p, n * 4, hipMemcpyDeviceToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(b.zero_rows.p, a.zero_rows.p, n, hipMemcpyDeviceToDevice, s));
+  }
+  UMAP_HIP(hipStreamSynchronize(s));
+  return m;
+}
+
+PrepareArgs prepare_args(const umapgpu_manifold& m, uint64_t total_epochs, const umapgpu_config& c, int32_t mt,
+                         uint64_t seed) {
+  PrepareArgs a;
+  a.total_epochs = total_epochs;
+  a.negative_sample_rate = c.negative_sample_rate;
+  a.repulsion_strength = c.repulsion_strength;  // optimizer.rs:81-83
+  a.learning_rate = c.learning_rate;
+  a.a = m.a;
+  a.b = m.b;
+  a.metric_type = mt;
+  a.seed = seed;
+  return a;
+}
+
+// Takes ownership of `m` only on success.
+std::unique_ptr<umapgpu_optimizer> make_optimizer(umapgpu_manifold* m, const float* init_dev, uint64_t n,
+                                                  uint64_t dim, uint64_t total_epochs, const umapgpu_config& c,
+                                                  int32_t mt, uint64_t seed) {
+  if (n != m->n_vertices)
+    panic("init has " + rs_u64(n) + " samples but the manifold has " + rs_u64(m->n_vertices) + " vertices");
+  if (dim < 1) panic("n_components must be >= 1, got 0");
+  if (mt != UMAPGPU_METRIC_EUCLIDEAN && mt != UMAPGPU_METRIC_GENERIC) panic("unknown metric type");
+  auto o = std::make_unique<umapgpu_optimizer>();
+  optimizer_prepare(m->g.graph, init_dev, dim, prepare_args(*m, total_epochs, c, mt, seed), &o->st, stream());
+  o->manifold.reset(m);
+  return o;
+}
+
+void check_overstep(const OptimizerState& st, uint64_t n) {
+  if (!(st.current_epoch + n <= st.total_epochs))  // optimizer.rs:298-304
+    throw Error(Err::Overstep, "Cannot step " + rs_u64(n) + " epochs: would exceed total_epochs " +
+                                   rs_u64(st.total_epochs) + " (current: " + rs_u64(st.current_epoch) + ")");
+}
+
+// Optimizer::step_epochs with MetricType::Generic: the reference's serial
+// host loop (layout/optimize_layout_generic.rs:35-124) through the caller's
+// distance callback, on host copies of the state.
+void step_generic(OptimizerState& st, uint64_t n_epochs, umapgpu_distance_fn dist, void* user) {
+  if (!dist) panic("MetricType::Generic needs a Metric::distance callback");
+  hipStream_t s = stream();
+  const uint64_t n = st.n, dim = st.dim, E = st.n_edges;
+  std::vector<uint32_t> head(E), tail(E);
+  std::vector<double> eps(E), eons(E), epns(E), eonns(E);
+  std::vector<float> H(n * dim);
+  optimizer_export(st, head.data(), tail.data(), eps.data(), eons.data(), epns.data(), eonns.data(), H.data(), s);
+  std::vector<float> T = H;  // optimizer.rs:314
+  std::vector<float> g1(dim), g2(dim);
+  const float a = st.a, b = st.b, gamma = st.gamma;
+  for (uint64_t epoch = st.current_epoch; epoch < st.current_epoch + n_epochs; ++epoch) {
+    const float alpha = st.initial_alpha * (1.0f - ((float)epoch / (float)st.total_epochs));
+    const uint64_t ek = rng_epoch_key(st.seed, epoch);
+    uint64_t npos = 0, nneg = 0;
+    for (uint64_t i = 0; i < E; ++i) {
+      if (!(eons[i] <= (double)epoch)) continue;
+      const uint32_t j = head[i], k = tail[i];
+      float* cur = H.data() + (uint64_t)j * dim;
+      float* oth = T.data() + (uint64_t)k * dim;
+      const float d = dist(cur, oth, dim, g1.data(), user);
+      (void)dist(oth, cur, dim, g2.data(), user);
+      const float w_l = (d > 0.0f) ? 1.0f / (1.0f + a * powf(d, 2.0f * b)) : 1.0f;
+      const float gc = ((2.0f * b) * (w_l - 1.0f)) / (d + 1e-6f);
+      for (uint64_t q = 0; q < dim; ++q) {
+        cur[q] = cur[q] + clip4(gc * g1[q]) * alpha;
+        oth[q] = oth[q] + clip4(gc * g2[q]) * alpha;
+      }
+      eons[i] = eons[i] + eps[i];
+      const uint32_t nn = sat_count(((double)epoch - eonns[i]) / epns[i]);
+      const uint64_t edk = rng_edge_key(ek, i);
+      for (uint32_t p = 0; p < nn; ++p) {
+        const uint32_t kk = rng_index(edk, p, (uint32_t)n);
+        if (kk == j) continue;
+        float* o2 = T.data() + (uint64_t)kk * dim;
+        const float dd = dist(cur, o2, dim, g1.data(), user);
+        const float wl = (dd > 0.0f) ? 1.0f / (1.0f + a * powf(dd, 2.0f * b)) : 1.0f;
+        const float gcn = (((gamma * 2.0f) * b) * wl) / (dd + 1e-6f);
+        for (uint64_t q = 0; q < dim; ++q) cur[q] = cur[q] + clip4(gcn * g1[q]) * alpha;
+      }
+      eonns[i] = eonns[i] + (double)nn * epns[i];
+      ++npos;
+      nneg += nn;
+    }
+    uint64_t stats[2];
+    UMAP_HIP(hipMemcpyAsync(stats, st.stats.p, 16, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    stats[0] += npos;
+    stats[1] += nneg;
+    UMAP_HIP(hipMemcpyAsync(st.stats.p, stats, 16, hipMemcpyHostToDevice, s));
+  }
+  // write the state back to the device
+  OptimizerState fresh;
+  optimizer_import(n, dim, E, head.data(), tail.data(), eps.data(), eons.data(), epns.data(), eonns.data(), H.data(),
+                   &fresh, s);
+  st.head_emb = std::move(fresh.head_emb);
+  st.sched = std::move(fresh.sched);
+  st.next = std::move(fresh.next);
+  st.current_epoch += n_epochs;
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+std::unique_ptr<umapgpu_fitted> into_fitted(std::unique_ptr<umapgpu_optimizer> o, const umapgpu_config& c) {
+  auto f = std::make_unique<umapgpu_fitted>();
+  f->n = o->st.n;
+  f->dim = o->st.dim;
+  f->embedding = std::move(o->st.head_emb);
+  f->manifold = std::move(o->manifold);
+  f->config = c;
+  return f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* umapgpu_version(void) { return "umapgpu 0.1.0 (gfx950; reference umap-rs 0.4.5)"; }
+const char* umapgpu_last_error(void) { return g_last_error.c_str(); }
+
+umapgpu_status umapgpu_set_device(int32_t device) {
+  return guarded([&] {
+    UMAP_HIP(hipSetDevice(device));
+    g_device = device;
+  });
+}
+
+umapgpu_status umapgpu_device_count(int32_t* count) {
+  return guarded([&] {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+  });
+}
+
+void umapgpu_config_default(umapgpu_config* c) {
+  std::memset(c, 0, sizeof *c);
+  c->n_components = 2;            // config.rs:222-233
+  c->min_dist = 0.1f;             // config.rs:45-51
+  c->spread = 1.0f;
+  c->n_neighbors = 15;            // config.rs:114-126
+  c->local_connectivity = 1.0f;
+  c->set_op_mix_ratio = 1.0f;
+  c->symmetrize = 1;
+  c->learning_rate = 1.0f;        // config.rs:170-179
+  c->negative_sample_rate = 5;
+  c->repulsion_strength = 1.0f;
+}
+
+umapgpu_status umapgpu_find_ab_params(float spread, float min_dist, float* a, float* b) {
+  return guarded([&] { find_ab(spread, min_dist, a, b); });
+}
+
+umapgpu_status umapgpu_learn_manifold(const umapgpu_config* cfg, uint64_t n_samples, const uint32_t* knn_indices,
+                                      uint64_t idx_rows, uint64_t idx_cols, uint64_t idx_stride,
+                                      const float* knn_dists, uint64_t dist_rows, uint64_t dist_cols,
+                                      uint64_t dist_stride, float metric_thr, umapgpu_manifold** out) {
+  return guarded([&] {
+    validate_parameters(*cfg, n_samples, idx_rows, idx_cols, dist_rows, dist_cols);
+    if (n_samples >= 0xffffffffull) panic("n_samples must be < 2^32 for u32 indices");
+    ensure_device();
+    hipStream_t s = stream();
+    const uint64_t n = dist_rows, k = dist_cols;
+    DevBuf di(n * k * 4), dd(n * k * 4);
+    upload_strided(di.p, knn_indices, n, k, idx_stride, 4, s);
+    upload_strided(dd.p, knn_dists, n, k, dist_stride, 4, s);
+    UMAP_HIP(hipStreamSynchronize(s));
+    *out = learn(*cfg, n_samples, di.as<uint32_t>(), dd.as<float>(), k, dist_stride == k, metric_thr).release();
+  });
+}
+
+umapgpu_status umapgpu_learn_manifold_device(const umapgpu_config* cfg, uint64_t n_samples, const uint32_t* idx,
+                                             const float* dists, uint64_t n_neighbors, float metric_thr,
+                                             umapgpu_manifold** out) {
+  return guarded([&] {
+    validate_parameters(*cfg, n_samples, n_samples, n_neighbors, n_samples, n_neighbors);
+    if (n_samples >= 0xffffffffull) panic("n_samples must be < 2^32 for u32 indices");
+    ensure_device();
+    *out = learn(*cfg, n_samples, idx, dists, n_neighbors, true, metric_thr).release();
+  });
+}
+
+umapgpu_status umapgpu_manifold_info(const umapgpu_manifold* m, uint64_t* nv, uint64_t* nnz, float* a, float* b) {
+  return guarded([&] {
+    if (nv) *nv = m->n_vertices;
+    if (nnz) *nnz = m->g.graph.nnz;
+    if (a) *a = m->a;
+    if (b) *b = m->b;
+  });
+}
+
+umapgpu_status umapgpu_manifold_copy_graph(const umapgpu_manifold* m, uint64_t* indptr, uint32_t* indices,
+                                           float* data) {
+  return guarded([&] {
+    ensure_device();
+    hipStream_t s = stream();
+    const DeviceCsr& g = m->g.graph;
+    if (indptr) UMAP_HIP(hipMemcpyAsync(indptr, g.indptr.p, (g.n + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (indices && g.nnz) UMAP_HIP(hipMemcpyAsync(indices, g.indices.p, g.nnz * 4, hipMemcpyDeviceToHost, s));
+    if (data && g.nnz) UMAP_HIP(hipMemcpyAsync(data, g.data.p, g.nnz * 4, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+  });
+}
+
+umapgpu_status umapgpu_manifold_copy_sigmas_rhos(const umapgpu_manifold* m, float* sigmas, float* rhos) {
+  return guarded([&] {
+    ensure_device();
+    hipStream_t s = stream();
+    const uint64_t n = m->n_vertices;
+    if (sigmas && n) UMAP_HIP(hipMemcpyAsync(sigmas, m->g.sigmas.p, n * 4, hipMemcpyDeviceToHost, s));
+    if (rhos && n) UMAP_HIP(hipMemcpyAsync(rhos, m->g.rhos.p, n * 4, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+  });
+}
+
+umapgpu_status umapgpu_manifold_disconnections(const umapgpu_manifold* m, uint64_t* er, uint64_t* vd) {
+  return guarded([&] {
+    if (er) *er = m->g.edges_removed;
+    if (vd) *vd = m->g.vertices_disconnected;
+  });
+}
+
+uint64_t umapgpu_manifold_timings_json(const umapgpu_manifold* m, char* buf, uint64_t len) {
+  std::string j = "{\"stages\": [";
+  for (size_t i = 0; i < m->g.timings.size(); ++i) {
+    char tmp[64];
+    snprintf(tmp, sizeof tmp, "%.6f", m->g.timings[i].ms);
+    j += (i ? ", " : "") + std::string("{\"name\": \"") + m->g.timings[i].name + "\", \"ms\": " + tmp + "}";
+  }
+  j += "], \"nnz_directed\": " + rs_u64(m->g.nnz_directed) + ", \"nnz\": " + rs_u64(m->g.graph.nnz) +
+       ", \"edges_removed\": " + rs_u64(m->g.edges_removed) +
+       ", \"vertices_disconnected\": " + rs_u64(m->g.vertices_disconnected) +
+       ", \"used_global_mean\": " + (m->g.used_global_mean ? "true" : "false") + ", \"warning\": \"";
+  for (char ch : m->warning) {
+    if (ch == '\n')
+      j += "\\n";
+    else if (ch == '"' || ch == '\\')
+      j += std::string("\\") + ch;
+    else
+      j += ch;
+  }
+  j += "\"}";
+  if (buf && len) {
+    const size_t c = std::min<size_t>(len - 1, j.size());
+    std::memcpy(buf, j.data(), c);
+    buf[c] = 0;
+  }
+  return j.size() + 1;
+}
+
+umapgpu_status umapgpu_manifold_from_host(uint64_t n, uint64_t nnz, const uint64_t* indptr, const uint32_t* indices,
+                                          const float* data, const float* sigmas, const float* rhos, float a, float b,
+                                          umapgpu_manifold** out) {
+  return guarded([&] {
+    if (indptr[0] != 0 || indptr[n] != nnz) panic("CSR indptr must start at 0 and end at nnz");
+    for (uint64_t r = 0; r < n; ++r) {
+      if (indptr[r + 1] < indptr[r]) panic("CSR indptr must be non-decreasing");
+      for (uint64_t e = indptr[r]; e < indptr[r + 1]; ++e) {
+        if (indices[e] >= n) panic("CSR column index out of range");
+        if (e > indptr[r] && indices[e] <= indices[e - 1]) panic("CSR rows must be sorted without duplicates");
+      }
+    }
+    ensure_device();
+    hipStream_t s = stream();
+    auto m = std::make_unique<umapgpu_manifold>();
+    m->n_vertices = n;
+    m->a = a;
+    m->b = b;
+    DeviceCsr& g = m->g.graph;
+    g.n = n;
+    g.nnz = nnz;
+    g.indptr = DevBuf((n + 1) * 8);
+    g.indices = DevBuf(std::max<uint64_t>(nnz, 1) * 4);
+    g.data = DevBuf(std::max<uint64_t>(nnz, 1) * 4);
+    m->g.sigmas = DevBuf(std::max<uint64_t>(n, 1) * 4);
+    m->g.rhos = DevBuf(std::max<uint64_t>(n, 1) * 4);
+    UMAP_HIP(hipMemcpyAsync(g.indptr.p, indptr, (n + 1) * 8, hipMemcpyHostToDevice, s));
+    if (nnz) {
+      UMAP_HIP(hipMemcpyAsync(g.indices.p, indices, nnz * 4, hipMemcpyHostToDevice, s));
+      UMAP_HIP(hipMemcpyAsync(g.data.p, data, nnz * 4, hipMemcpyHostToDevice, s));
+    }
+    if (n && sigmas) UMAP_HIP(hipMemcpyAsync(m->g.sigmas.p, sigmas, n * 4, hipMemcpyHostToDevice, s));
+    if (n && rhos) UMAP_HIP(hipMemcpyAsync(m->g.rhos.p, rhos, n * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    device_zero_rows(g, &m->g.zero_rows, &m->g.vertices_disconnected, s);
+    *out = m.release();
+  });
+}
+
+umapgpu_status umapgpu_manifold_clone(const umapgpu_manifold* m, umapgpu_manifold** out) {
+  return guarded([&] {
+    ensure_device();
+    *out = clone_manifold(*m).release();
+  });
+}
+
+void umapgpu_manifold_destroy(umapgpu_manifold* m) { delete m; }
+
+umapgpu_status umapgpu_optimizer_new(umapgpu_manifold* manifold, const float* init, uint64_t n, uint64_t dim,
+                                     uint64_t total_epochs, const umapgpu_config* cfg, int32_t mt, uint64_t seed,
+                                     umapgpu_optimizer** out) {
+  return guarded([&] {
+    if (n != manifold->n_vertices)
+      panic("init has " + rs_u64(n) + " samples but the manifold has " + rs_u64(manifold->n_vertices) +
+            " vertices");
+    ensure_device();
+    hipStream_t s = stream();
+    DevBuf d(std::max<uint64_t>(n * dim, 1) * 4);
+    if (n * dim) UMAP_HIP(hipMemcpyAsync(d.p, init, n * dim * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    *out = make_optimizer(manifold, d.as<float>(), n, dim, total_epochs, *cfg, mt, seed).release();
+  });
+}
+
+umapgpu_status umapgpu_optimizer_new_device(umapgpu_manifold* manifold, const float* init_dev, uint64_t n,
+                                            uint64_t dim, uint64_t total_epochs, const umapgpu_config* cfg,
+                                            int32_t mt, uint64_t seed, umapgpu_optimizer** out) {
+  return guarded([&] {
+    ensure_device();
+    *out = make_optimizer(manifold, init_dev, n, dim, total_epochs, *cfg, mt, seed).release();
+  });
+}
+
+umapgpu_status umapgpu_optimizer_step_epochs(umapgpu_optimizer* o, uint64_t n) {
+  return guarded([&] {
+    check_overstep(o->st, n);
+    if (o->st.metric_type == UMAPGPU_METRIC_GENERIC)
+      panic("MetricType::Generic needs a Metric::distance callback (umapgpu_optimizer_step_epochs_generic)");
+    ensure_device();
+    optimizer_step(&o->st, n, stream());
+  });
+}
+
+umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* o, uint64_t n, umapgpu_distance_fn dist,
+                                                     void* user) {
+  return guarded([&] {
+    check_overstep(o->st, n);
+    ensure_device();
+    step_generic(o->st, n, dist, user);
+  });
+}
+
+umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* o, int32_t mode) {
+  return guarded([&] {
+    if (mode < 0 || mode > 2) panic("unknown SGD mode");
+    o->st.mode = (SgdMode)mode;
+  });
+}
+
+uint64_t umapgpu_optimizer_current_epoch(const umapgpu_optimizer* o) { return o->st.current_epoch; }
+uint64_t umapgpu_optimizer_total_epochs(const umapgpu_optimizer* o) { return o->st.total_epochs; }
+uint64_t umapgpu_optimizer_remaining_epochs(const umapgpu_optimizer* o) {
+  return o->st.total_epochs - o->st.current_epoch;
+}
+
+umapgpu_status umapgpu_optimizer_copy_embedding(const umapgpu_optimizer* o, float* out) {
+  return guarded([&] {
+    ensure_device();
+    optimizer_export(o->st, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, out, stream());
+  });
+}
+
+const float* umapgpu_optimizer_embedding_device(const umapgpu_optimizer* o) { return o->st.head_emb.as<float>(); }
+const umapgpu_manifold* umapgpu_optimizer_manifold(const umapgpu_optimizer* o) { return o->manifold.get(); }
+
+umapgpu_status umapgpu_optimizer_info(const umapgpu_optimizer* o, uint64_t* ne, uint64_t* nv, uint64_t* dim) {
+  return guarded([&] {
+    if (ne) *ne = o->st.n_edges;
+    if (nv) *nv = o->st.n;
+    if (dim) *dim = o->st.dim;
+  });
+}
+
+umapgpu_status umapgpu_optimizer_stats(const umapgpu_optimizer* o, uint64_t* pos, uint64_t* neg, double* ms) {
+  return guarded([&] {
+    ensure_device();
+    optimizer_read_stats(o->st, pos, neg, stream());
+    if (ms) *ms = o->st.last_step_ms;
+  });
+}
+
+umapgpu_status umapgpu_optimizer_export_state(const umapgpu_optimizer* o, uint32_t* head, uint32_t* tail,
+                                              double* eps, double* eons, double* epns, double* eonns,
+                                              float* embedding) {
+  return guarded([&] {
+    ensure_device();
+    optimizer_export(o->st, head, tail, eps, eons, epns, eonns, embedding, stream());
+  });
+}
+
+umapgpu_status umapgpu_optimizer_export_params(const umapgpu_optimizer* o, uint64_t* cur, uint64_t* tot, float* g,
+                                               float* ia, double* nsr, int32_t* mt, uint64_t* seed) {
+  return guarded([&] {
+    if (cur) *cur = o->st.current_epoch;
+    if (tot) *tot = o->st.total_epochs;
+    if (g) *g = o->st.gamma;
+    if (ia) *ia = o->st.initial_alpha;
+    if (nsr) *nsr = o->st.negative_sample_rate;
+    if (mt) *mt = o->st.metric_type;
+    if (seed) *seed = o->st.seed;
+  });
+}
+
+umapgpu_status umapgpu_optimizer_import_state(umapgpu_manifold* manifold, uint64_t n_edges, const uint32_t* head,
+                                              const uint32_t* tail, const double* eps, const double* eons,
+                                              const double* epns, const double* eonns, const float* embedding,
+                                              uint64_t dim, uint64_t cur, uint64_t tot, float gamma, float ia,
+                                              double nsr, int32_t mt, uint64_t seed, umapgpu_optimizer** out) {
+  return guarded([&] {
+    if (cur > tot) panic("checkpoint current_epoch exceeds total_epochs");
+    ensure_device();
+    auto o = std::make_unique<umapgpu_optimizer>();
+    OptimizerState& st = o->st;
+    st.total_epochs = tot;
+    st.current_epoch = cur;
+    st.a = manifold->a;
+    st.b = manifold->b;
+    st.gamma = gamma;
+    st.initial_alpha = ia;
+    st.negative_sample_rate = nsr;
+    st.metric_type = mt;
+    st.seed = seed;
+    optimizer_import(manifold->n_vertices, dim, n_edges, head, tail, eps, eons, epns, eonns, embedding, &st,
+                     stream());
+    o->manifold.reset(manifold);
+    *out = o.release();
+  });
+}
+
+umapgpu_status umapgpu_optimizer_into_fitted(umapgpu_optimizer* o, const umapgpu_config* cfg, umapgpu_fitted** out) {
+  return guarded([&] {
+    std::unique_ptr<umapgpu_optimizer> own(o);
+    *out = into_fitted(std::move(own), *cfg).release();
+  });
+}
+
+void umapgpu_optimizer_destroy(umapgpu_optimizer* o) { delete o; }
+
+umapgpu_status umapgpu_fit(const umapgpu_config* cfg, uint64_t n_samples, const uint32_t* knn_indices,
+                           uint64_t idx_rows, uint64_t idx_cols, uint64_t idx_stride, const float* knn_dists,
+                           uint64_t dist_rows, uint64_t dist_cols, uint64_t dist_stride, const float* init,
+                           uint64_t init_rows, uint64_t init_cols, float metric_thr, int32_t out_mt,
+                           umapgpu_distance_fn out_dist, void* out_user, uint64_t seed, umapgpu_fitted** out) {
+  return guarded([&] {
+    const umapgpu_config& c = *cfg;
+    if (init_cols != c.n_components)  // embedding.rs:284-298
+      panic("init has " + rs_u64(init_cols) + " components but n_components is " + rs_u64(c.n_components));
+    if (init_rows != n_samples)
+      panic("init has " + rs_u64(init_rows) + " samples but data has " + rs_u64(n_samples) + " samples");
+    validate_parameters(c, n_samples, idx_rows, idx_cols, dist_rows, dist_cols);
+    if (n_samples >= 0xffffffffull) panic("n_samples must be < 2^32 for u32 indices");
+    if (out_mt == UMAPGPU_METRIC_GENERIC && !out_dist) panic("MetricType::Generic needs a Metric::distance callback");
+    ensure_device();
+    hipStream_t s = stream();
+    const uint64_t n = dist_rows, k = dist_cols;
+    DevBuf di(n * k * 4), dd(n * k * 4), dinit(std::max<uint64_t>(init_rows * init_cols, 1) * 4);
+    upload_strided(di.p, knn_indices, n, k, idx_stride, 4, s);
+    upload_strided(dd.p, knn_dists, n, k, dist_stride, 4, s);
+    if (init_rows * init_cols)
+      UMAP_HIP(hipMemcpyAsync(dinit.p, init, init_rows * init_cols * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    auto m = learn(c, n_samples, di.as<uint32_t>(), dd.as<float>(), k, dist_stride == k, metric_thr);
+    di.release();
+    dd.release();
+    const uint64_t total = c.has_n_epochs ? c.n_epochs : (n_samples <= 10000 ? 500 : 200);  // :304-308
+    auto o = make_optimizer(m.get(), dinit.as<float>(), init_rows, init_cols, total, c, out_mt, seed);
+    (void)m.release();  // now owned by the optimizer
+    if (out_mt == UMAPGPU_METRIC_GENERIC)
+      step_generic(o->st, total, out_dist, out_user);
+    else
+      optimizer_step(&o->st, total, s);
+    auto f = into_fitted(std::move(o), c);
+    if (f->manifold->g.vertices_disconnected && f->n * f->dim) {  // :327-334
+      nan_rows_kernel<<<(unsigned)ceil_div(f->n * f->dim, 256), 256, 0, s>>>(
+          f->embedding.as<float>(), f->manifold->g.zero_rows.as<uint8_t>(), f->n, f->dim);
+      UMAP_LAUNCH_CHECK();
+      UMAP_HIP(hipStreamSynchronize(s));
+    }
+    *out = f.release();
+  });
+}
+
+umapgpu_status umapgpu_fitted_info(const umapgpu_fitted* f, uint64_t* n, uint64_t* dim) {
+  return guarded([&] {
+    if (n) *n = f->n;
+    if (dim) *dim = f->dim;
+  });
+}
+
+umapgpu_status umapgpu_fitted_copy_embedding(const umapgpu_fitted* f, float* out) {
+  return guarded([&] {
+    ensure_device();
+    hipStream_t s = stream();
+    if (f->n * f->dim) UMAP_HIP(hipMemcpyAsync(out, f->embedding.p, f->n * f->dim * 4, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+  });
+}
+
+const umapgpu_manifold* umapgpu_fitted_manifold(const umapgpu_fitted* f) { return f->manifold.get(); }
+
+umapgpu_status umapgpu_fitted_config(const umapgpu_fitted* f, umapgpu_config* cfg) {
+  return guarded([&] { *cfg = f->config; });
+}
+
+umapgpu_status umapgpu_fitted_from_host(umapgpu_manifold* manifold, const float* embedding, uint64_t n, uint64_t dim,
+                                        const umapgpu_config* cfg, umapgpu_fitted** out) {
+  return guarded([&] {
+    if (n != manifold->n_vertices) panic("embedding rows must equal the manifold's n_vertices");
+    ensure_device();
+    hipStream_t s = stream();
+    auto f = std::make_unique<umapgpu_fitted>();
+    f->n = n;
+    f->dim = dim;
+    f->embedding = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
+    if (n * dim) UMAP_HIP(hipMemcpyAsync(f->embedding.p, embedding, n * dim * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    f->manifold.reset(manifold);
+    f->config = *cfg;
+    *out = f.release();
+  });
+}
+
+void umapgpu_fitted_destroy(umapgpu_fitted* f) { delete f; }
+
+umapgpu_status umapgpu_device_alloc(uint64_t bytes, void** ptr) {
+  return guarded([&] {
+    ensure_device();
+    void* p = nullptr;
+    UMAP_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    *ptr = p;
+  });
+}
+
+umapgpu_status umapgpu_device_free(void* ptr) {
+  return guarded([&] { UMAP_HIP(hipFree(ptr)); });
+}
+
+umapgpu_status umapgpu_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    ensure_device();
+    UMAP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+umapgpu_status umapgpu_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    ensure_device();
+    UMAP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+umapgpu_status umapgpu_synchronize(void) {
+  return guarded([&] {
+    ensure_device();
+    UMAP_HIP(hipDeviceSynchronize());
+  });
+}
+
+umapgpu_status umapgpu_knn_bruteforce_device(const float* data, uint64_t n, uint64_t dim, uint64_t k, uint32_t* idx,
+                                             float* dists) {
+  return guarded([&] {
+    ensure_device();
+    knn_bruteforce(data, n, dim, k, idx, dists, stream());
+  });
+}
+
+umapgpu_status umapgpu_debug_copy_device(const void* src, void* dst, uint64_t bytes) {
+  return guarded([&] {
+    ensure_device();
+    device_copy_calibration(src, dst, bytes, stream());
+  });
+}
+
+umapgpu_status umapgpu_debug_expf_device(const float* x, float* y, uint64_t n) {
+  return guarded([&] {
+    ensure_device();
+    device_expf(x, y, n, stream());
+  });
+}
+
+}  // extern "C"

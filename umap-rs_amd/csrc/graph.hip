@@ -1,0 +1,623 @@
+// Graph construction kernels: smooth-kNN sigma/rho, membership strengths,
+// COO transpose by radix sort, and the fuzzy union/intersection merge.
+//
+// Reference stages (umap-rs 0.4.5):
+//   disconnection set      embedding.rs:168-190
+//   smooth_knn_dist        umap/smooth_knn_dist.rs:60-198
+//   build_membership_csr   umap/fuzzy_simplicial_set.rs:171-306
+//   build_csc_structure    umap/fuzzy_simplicial_set.rs:310-370
+//   apply_set_operations   umap/fuzzy_simplicial_set.rs:392-552
+//   empty-row count        embedding.rs:215-222
+//
+// All arithmetic follows the reference's f32 operation order with no FMA
+// contraction (the library is built with -ffp-contract=off) and the
+// glibc-equivalent expf, so sigma/rho, the CSR structure and every weight are
+// bit-identical to the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include "graph.hpp"
+#include "libm_expf.hpp"
+#include "primitives.hpp"
+
+namespace umapgpu {
+
+namespace {
+
+constexpr float kSmoothKTolerance = 1e-5f;  // umap/constants.rs:4
+constexpr float kMinKDistScale = 1e-3f;     // umap/constants.rs:7
+
+// Rows per block for lane-per-row kernels that stage `words` 32-bit words
+// per kNN slot into LDS (row stride padded to an odd word count so that the
+// 64 lanes of a wave hit distinct banks).
+uint32_t rows_per_block(uint32_t k, uint32_t words) {
+  const uint32_t ks = k | 1u;
+  for (uint32_t r = 256; r >= 16; r >>= 1)
+    if ((uint64_t)r * ks * 4u * words <= 65536u) return r;
+  throw Error(Err::Unsupported, "n_neighbors too large for the device graph kernels (max 1023)");
+}
+
+// Coalesced copy of rows [row0, row0+nrows) x k into LDS with stride ks.
+template <typename T>
+__device__ inline void stage_rows(const T* __restrict__ src, uint64_t row0, uint32_t nrows, uint32_t k, uint32_t ks,
+                                  T* sh) {
+  const uint32_t tot = nrows * k;
+  const T* base = src + row0 * k;
+  for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x) {
+    const uint32_t r = t / k, j = t - r * k;
+    sh[r * ks + j] = base[t];
+  }
+}
+
+// ndarray's eight-accumulator unrolled fold (the order of `mean()`).
+__device__ inline float unrolled_sum(const float* xs, uint32_t len) {
+  float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f, p5 = 0.f, p6 = 0.f, p7 = 0.f;
+  while (len >= 8) {
+    p0 = p0 + xs[0];
+    p1 = p1 + xs[1];
+    p2 = p2 + xs[2];
+    p3 = p3 + xs[3];
+    p4 = p4 + xs[4];
+    p5 = p5 + xs[5];
+    p6 = p6 + xs[6];
+    p7 = p7 + xs[7];
+    xs += 8;
+    len -= 8;
+  }
+  float acc = 0.0f;
+  acc = acc + (p0 + p4);
+  acc = acc + (p1 + p5);
+  acc = acc + (p2 + p6);
+  acc = acc + (p3 + p7);
+  for (uint32_t i = 0; i < len && i < 7; ++i) acc = acc + xs[i];
+  return acc;
+}
+
+struct SmoothArgs {
+  float target;
+  int32_t n_iter;
+  uint64_t local_conn_usize;  // local_connectivity as usize
+  uint64_t index;             // floor(local_connectivity) as usize
+  float interpolation;        // local_connectivity - floor(local_connectivity)
+};
+
+// One lane computes one row (smooth_knn_dist.rs:81-185).  The psum is the
+// reference's serial sum over slots 1..k, so sigma is bit-identical.
+__global__ void sigma_rho_kernel(const float* __restrict__ dists, uint64_t n, uint32_t k, SmoothArgs a,
+                                 float* __restrict__ sig_out, float* __restrict__ rho_out,
+                                 uint32_t* __restrict__ need_global_mean) {
+  extern __shared__ float sh[];
+  const uint32_t R = blockDim.x, ks = k | 1u;
+  const uint64_t row0 = (uint64_t)blockIdx.x * R;
+  const uint32_t nrows = (uint32_t)((n - row0) < R ? (n - row0) : R);
+  stage_rows(dists, row0, nrows, k, ks, sh);
+  __syncthreads();
+  if (threadIdx.x >= nrows) return;
+  const float* d = sh + threadIdx.x * ks;
+
+  uint64_t non_zero_count = 0;
+  float max_non_zero = 0.0f;
+  for (uint32_t j = 0; j < k; ++j) {
+    const float v = d[j];
+    if (v > 0.0f) {
+      ++non_zero_count;
+      if (v > max_non_zero) max_non_zero = v;
+    }
+  }
+  float rho = 0.0f;
+  if (non_zero_count >= a.local_conn_usize) {
+    float nth_minus_1 = 0.0f, nth = 0.0f;
+    uint64_t count = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+      const float v = d[j];
+      if (v > 0.0f) {
+        if (count + 1 == a.index) nth_minus_1 = v;
+        if (count + 1 == a.index + 1) nth = v;
+        ++count;
+        if (count > a.index + 1) break;
+      }
+    }
+    if (a.index > 0) {
+      rho = nth_minus_1;
+      if (a.interpolation > kSmoothKTolerance) rho = rho + a.interpolation * (nth - nth_minus_1);
+    } else {
+      float first_non_zero = 0.0f;
+      for (uint32_t j = 0; j < k; ++j)
+        if (d[j] > 0.0f) {
+          first_non_zero = d[j];
+          break;
+        }
+      rho = a.interpolation * first_non_zero;
+    }
+  } else if (non_zero_count > 0) {
+    rho = max_non_zero;
+  }
+
+  float lo = 0.0f, hi = INFINITY, mid = 1.0f;
+  for (int32_t it = 0; it < a.n_iter; ++it) {
+    float psum = 0.0f;
+    for (uint32_t j = 1; j < k; ++j) {
+      const float t = d[j] - rho;
+      if (t > 0.0f)
+        psum = psum + libm_expf(-(t / mid));
+      else
+        psum = psum + 1.0f;
+    }
+    if (fabsf(psum - a.target) < kSmoothKTolerance) break;
+    if (psum > a.target) {
+      hi = mid;
+      mid = (lo + hi) / 2.0f;
+    } else {
+      lo = mid;
+      if (hi == INFINITY)
+        mid = mid * 2.0f;
+      else
+        mid = (lo + hi) / 2.0f;
+    }
+  }
+  float sigma = mid;
+  if (rho > 0.0f) {
+    const float mean_ith = unrolled_sum(d, k) / (float)k;
+    if (sigma < kMinKDistScale * mean_ith) sigma = kMinKDistScale * mean_ith;
+  } else {
+    // floor against the global mean is applied by sigma_floor_kernel, which
+    // only runs when some row lands here
+    atomicOr(need_global_mean, 1u);
+  }
+  const uint64_t row = row0 + threadIdx.x;
+  sig_out[row] = sigma;
+  rho_out[row] = rho;
+}
+
+// Global mean in ndarray's order for a contiguous array: eight sequential
+// accumulator chains (lanes 0..7), combined as the unrolled fold does.
+// Rare path: only needed when a row has rho <= 0.
+__global__ void unrolled_mean_kernel(const float* __restrict__ x, uint64_t len, float* __restrict__ out) {
+  __shared__ float part[8];
+  const uint32_t l = threadIdx.x;
+  const uint64_t nfull = len / 8;
+  if (l < 8) {
+    float p = 0.0f;
+    uint64_t b = 0;
+    for (; b + 4 <= nfull; b += 4) {
+      const float v0 = x[(b + 0) * 8 + l], v1 = x[(b + 1) * 8 + l], v2 = x[(b + 2) * 8 + l], v3 = x[(b + 3) * 8 + l];
+      p = p + v0;
+      p = p + v1;
+      p = p + v2;
+      p = p + v3;
+    }
+    for (; b < nfull; ++b) p = p + x[b * 8 + l];
+    part[l] = p;
+  }
+  __syncthreads();
+  if (l == 0) {
+    float acc = 0.0f;
+    acc = acc + (part[0] + part[4]);
+    acc = acc + (part[1] + part[5]);
+    acc = acc + (part[2] + part[6]);
+    acc = acc + (part[3] + part[7]);
+    const uint64_t rem = len - nfull * 8;
+    for (uint64_t i = 0; i < rem && i < 7; ++i) acc = acc + x[nfull * 8 + i];
+    *out = acc / (float)len;
+  }
+}
+
+// Strided caller layouts: ndarray folds each row and adds the row sums in
+// order.  Row sums are independent; the final chain is sequential.
+__global__ void row_unrolled_sums_kernel(const float* __restrict__ x, uint64_t n, uint32_t k, float* __restrict__ rs) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) rs[r] = unrolled_sum(x + r * k, k);
+}
+__global__ void sequential_mean_kernel(const float* __restrict__ rs, uint64_t n, uint64_t count, float* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float s = 0.0f;
+  for (uint64_t i = 0; i < n; ++i) s = s + rs[i];
+  *out = s / (float)count;
+}
+
+__global__ void sigma_floor_kernel(float* __restrict__ sig, const float* __restrict__ rho, uint64_t n,
+                                   const float* __restrict__ mean) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if (!(rho[r] > 0.0f)) {
+    const float floor_v = kMinKDistScale * *mean;
+    if (sig[r] < floor_v) sig[r] = floor_v;
+  }
+}
+
+__global__ void count_disconnected_kernel(const float* __restrict__ dists, uint64_t total, float thr,
+                                          unsigned long long* __restrict__ out) {
+  uint64_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x)
+    c += (dists[i] >= thr) ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+// compute_membership_strength (fuzzy_simplicial_set.rs:294-306)
+__device__ inline float membership(float d, float rho, float sigma) {
+  if (d - rho <= 0.0f || sigma == 0.0f) return 1.0f;
+  return libm_expf(-(d - rho) / sigma);
+}
+
+// Kept-slot count per row (fuzzy_simplicial_set.rs:182-202): skip
+// disconnected slots, self loops, sentinel indices and zero weights.
+__global__ void memb_count_kernel(const uint32_t* __restrict__ idx, const float* __restrict__ dists, uint64_t n,
+                                  uint32_t k, float thr, const float* __restrict__ sig, const float* __restrict__ rho,
+                                  uint32_t* __restrict__ cnt) {
+  extern __shared__ uint32_t shu[];
+  const uint32_t R = blockDim.x, ks = k | 1u;
+  uint32_t* si = shu;
+  float* sd = reinterpret_cast<float*>(shu + R * ks);
+  const uint64_t row0 = (uint64_t)blockIdx.x * R;
+  const uint32_t nrows = (uint32_t)((n - row0) < R ? (n - row0) : R);
+  stage_rows(idx, row0, nrows, k, ks, si);
+  stage_rows(dists, row0, nrows, k, ks, sd);
+  __syncthreads();
+  if (threadIdx.x >= nrows) return;
+  const uint64_t i = row0 + threadIdx.x;
+  const float r = rho[i], s = sig[i];
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < k; ++j) {
+    const float d = sd[threadIdx.x * ks + j];
+    if (d >= thr) continue;
+    const uint64_t col = si[threadIdx.x * ks + j];
+    if (col == i || col >= n) continue;
+    if (membership(d, r, s) != 0.0f) ++c;
+  }
+  cnt[i] = c;
+}
+
+// Fill + per-row stable insertion sort by column (fuzzy_simplicial_set.rs:
+// 232-282).  The kept entries are compacted in place in the lane's LDS row
+// (write index <= read index), sorted there, then written to the CSR.
+__global__ void memb_fill_kernel(const uint32_t* __restrict__ idx, const float* __restrict__ dists, uint64_t n,
+                                 uint32_t k, float thr, const float* __restrict__ sig, const float* __restrict__ rho,
+                                 const uint64_t* __restrict__ indptr, uint32_t* __restrict__ out_idx,
+                                 float* __restrict__ out_w, uint32_t* __restrict__ out_row) {
+  extern __shared__ uint32_t shu[];
+  const uint32_t R = blockDim.x, ks = k | 1u;
+  uint32_t* si = shu;
+  float* sd = reinterpret_cast<float*>(shu + R * ks);
+  const uint64_t row0 = (uint64_t)blockIdx.x * R;
+  const uint32_t nrows = (uint32_t)((n - row0) < R ? (n - row0) : R);
+  stage_rows(idx, row0, nrows, k, ks, si);
+  stage_rows(dists, row0, nrows, k, ks, sd);
+  __syncthreads();
+  if (threadIdx.x >= nrows) return;
+  const uint64_t i = row0 + threadIdx.x;
+  uint32_t* ci = si + threadIdx.x * ks;
+  float* cw = sd + threadIdx.x * ks;
+  const float r = rho[i], s = sig[i];
+  uint32_t o = 0;
+  for (uint32_t j = 0; j < k; ++j) {
+    const float d = cw[j];
+    const uint32_t col = ci[j];
+    if (d >= thr) continue;
+    if ((uint64_t)col == i || (uint64_t)col >= n) continue;
+    const float v = membership(d, r, s);
+    if (v != 0.0f) {
+      ci[o] = col;
+      cw[o] = v;
+      ++o;
+    }
+  }
+  for (uint32_t a = 1; a < o; ++a) {
+    const uint32_t kc = ci[a];
+    const float kv = cw[a];
+    uint32_t m = a;
+    while (m > 0 && ci[m - 1] > kc) {
+      ci[m] = ci[m - 1];
+      cw[m] = cw[m - 1];
+      --m;
+    }
+    ci[m] = kc;
+    cw[m] = kv;
+  }
+  const uint64_t base = indptr[i];
+  for (uint32_t a = 0; a < o; ++a) {
+    out_idx[base + a] = ci[a];
+    out_w[base + a] = cw[a];
+    if (out_row) out_row[base + a] = (uint32_t)i;
+  }
+}
+
+// Aᵀ row pointers from the column-sorted COO: tptr[c] = first position whose
+// key is >= c.  Each c is written exactly once.
+__global__ void transpose_ptr_kernel(const uint32_t* __restrict__ keys, uint64_t e, uint64_t n,
+                                     uint64_t* __restrict__ tptr) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= e) return;
+  const int64_t prev = (i == 0) ? -1 : (int64_t)keys[i - 1];
+  const int64_t cur = (int64_t)keys[i];
+  for (int64_t c = prev + 1; c <= cur; ++c) tptr[c] = i;
+  if (i == e - 1)
+    for (uint64_t c = (uint64_t)cur + 1; c <= n; ++c) tptr[c] = e;
+}
+
+// mix*a + mix*b + (1-2mix)*a*b, left to right, each op rounded
+// (fuzzy_simplicial_set.rs:422-423, :487-488).
+__device__ inline float fuzzy_union(float mix, float prod, float a, float b) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(mix, a), __fmul_rn(mix, b)), __fmul_rn(__fmul_rn(prod, a), b));
+}
+
+// Row r of mix(A+Aᵀ) + (1-2mix)A∘Aᵀ as a merge of two column-sorted lists:
+// A's row r (direct entries) and Aᵀ's row r (rows c with A[c,r]).  Entries
+// present in both take the direct formula with val_cr = A[c,r]; direct-only
+// entries use val_cr = 0; transpose-only entries are mix*A[c,r] (:438, :506).
+// The output is already column-sorted, so the reference's final row sort
+// (:524-542) is the identity here.
+template <bool FILL>
+__global__ void merge_kernel(const uint64_t* __restrict__ aptr, const uint32_t* __restrict__ aind,
+                             const float* __restrict__ adat, const uint64_t* __restrict__ tptr,
+                             const uint32_t* __restrict__ trow, const float* __restrict__ tdat, uint64_t n, float mix,
+                             float prod, uint32_t* __restrict__ cnt, const uint64_t* __restrict__ optr,
+                             uint32_t* __restrict__ oind, float* __restrict__ odat) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint64_t ia = aptr[r];
+  const uint64_t a1 = aptr[r + 1];
+  uint64_t it = tptr[r];
+  const uint64_t t1 = tptr[r + 1];
+  uint64_t o = FILL ? optr[r] : 0;
+  uint32_t c = 0;
+  while (ia < a1 || it < t1) {
+    const uint32_t ca = (ia < a1) ? aind[ia] : 0xffffffffu;
+    const uint32_t ct = (it < t1) ? trow[it] : 0xffffffffu;
+    uint32_t col;
+    float f;
+    if (ca < ct) {
+      col = ca;
+      f = fuzzy_union(mix, prod, adat[ia], 0.0f);
+      ++ia;
+    } else if (ct < ca) {
+      col = ct;
+      f = __fmul_rn(mix, tdat[it]);
+      ++it;
+    } else {
+      col = ca;
+      f = fuzzy_union(mix, prod, adat[ia], tdat[it]);
+      ++ia;
+      ++it;
+    }
+    if (f != 0.0f) {
+      if (FILL) {
+        oind[o] = col;
+        odat[o] = f;
+        ++o;
+      } else {
+        ++c;
+      }
+    }
+  }
+  if (!FILL) cnt[r] = c;
+}
+
+// Rows whose weights sum (sequential f32 fold, embedding.rs:219) to 0.
+__global__ void zero_rows_kernel(const uint64_t* __restrict__ ptr, const float* __restrict__ dat, uint64_t n,
+                                 uint8_t* __restrict__ flags, unsigned long long* __restrict__ count) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t z = 0;
+  if (r < n) {
+    float s = -0.0f;
+    for (uint64_t e = ptr[r]; e < ptr[r + 1]; ++e) s = s + dat[e];
+    z = (s == 0.0f) ? 1u : 0u;
+    flags[r] = (uint8_t)z;
+  }
+  for (int off = 32; off > 0; off >>= 1) z += __shfl_xor(z, off, 64);
+  if ((threadIdx.x & 63) == 0 && z) atomicAdd(count, (unsigned long long)z);
+}
+
+// Plain streaming copy of known size: the reference point that scales the
+// FETCH_SIZE / WRITE_SIZE counters into bytes.
+__global__ void copy_calibration_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void expf_kernel(const float* __restrict__ x, float* __restrict__ y, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] = libm_expf(x[i]);
+}
+
+unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)ceil_div(n, block); }
+
+struct EventTimer {
+  hipStream_t s;
+  std::vector<std::pair<std::string, hipEvent_t>> marks;
+  explicit EventTimer(hipStream_t st) : s(st) { mark("start"); }
+  void mark(const std::string& name) {
+    hipEvent_t e;
+    UMAP_HIP(hipEventCreate(&e));
+    UMAP_HIP(hipEventRecord(e, s));
+    marks.emplace_back(name, e);
+  }
+  void collect(std::vector<StageTime>* out) {
+    UMAP_HIP(hipEventSynchronize(marks.back().second));
+    for (size_t i = 1; i < marks.size(); ++i) {
+      float ms = 0.0f;
+      UMAP_HIP(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
+      out->push_back({marks[i].first, (double)ms});
+    }
+    float tot = 0.0f;
+    UMAP_HIP(hipEventElapsedTime(&tot, marks.front().second, marks.back().second));
+    out->push_back({"total", (double)tot});
+  }
+  ~EventTimer() {
+    for (auto& m : marks) (void)hipEventDestroy(m.second);
+  }
+};
+
+}  // namespace
+
+void build_graph(const uint32_t* idx, const float* dists, const GraphParams& p, GraphResult* out, hipStream_t s) {
+  const uint64_t n = p.n;
+  const uint32_t k = p.k;
+  if (n >= 0xffffffffull) throw Error(Err::InvalidArgument, "n_samples must be < 2^32 for u32 indices");
+  EventTimer timer(s);
+  out->sigmas = DevBuf(n * 4);
+  out->rhos = DevBuf(n * 4);
+
+  // disconnection detection (embedding.rs:177-186): the DashSet's size
+  DevBuf ctr(8);
+  UMAP_HIP(hipMemsetAsync(ctr.p, 0, 8, s));
+  if (n * k) {
+    count_disconnected_kernel<<<(unsigned)std::min<uint64_t>(grid_for(n * k, 256), 4096), 256, 0, s>>>(
+        dists, n * k, p.disconnection_distance, ctr.as<unsigned long long>());
+    UMAP_LAUNCH_CHECK();
+  }
+  timer.mark("disconnection detection");
+
+  // smooth_knn_dist
+  SmoothArgs sa;
+  sa.target = p.target;
+  sa.n_iter = p.n_iter;
+  const float lc = p.local_connectivity;
+  sa.local_conn_usize = (lc > 0.0f) ? (uint64_t)lc : 0;
+  const float lcf = floorf(lc);
+  sa.index = (lcf > 0.0f) ? (uint64_t)lcf : 0;
+  sa.interpolation = lc - lcf;
+  DevBuf flag(4);
+  UMAP_HIP(hipMemsetAsync(flag.p, 0, 4, s));
+  if (n) {
+    const uint32_t R = rows_per_block(k, 1);
+    sigma_rho_kernel<<<grid_for(n, R), R, (size_t)R * (k | 1u) * 4, s>>>(dists, n, k, sa, out->sigmas.as<float>(),
+                                                                          out->rhos.as<float>(), flag.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
+  uint32_t need_global = 0;
+  UMAP_HIP(hipMemcpyAsync(&need_global, flag.p, 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  out->used_global_mean = need_global != 0;
+  if (need_global) {
+    DevBuf mean(4);
+    if (p.contiguous_input || n == 1) {
+      unrolled_mean_kernel<<<1, 64, 0, s>>>(dists, n * k, mean.as<float>());
+    } else {
+      DevBuf rs(n * 4);
+      row_unrolled_sums_kernel<<<grid_for(n, 256), 256, 0, s>>>(dists, n, k, rs.as<float>());
+      sequential_mean_kernel<<<1, 64, 0, s>>>(rs.as<float>(), n, n * k, mean.as<float>());
+      UMAP_HIP(hipStreamSynchronize(s));
+    }
+    UMAP_LAUNCH_CHECK();
+    sigma_floor_kernel<<<grid_for(n, 256), 256, 0, s>>>(out->sigmas.as<float>(), out->rhos.as<float>(), n,
+                                                         mean.as<float>());
+    UMAP_LAUNCH_CHECK();
+    UMAP_HIP(hipStreamSynchronize(s));
+  }
+  timer.mark("smooth_knn_dist");
+
+  // membership CSR (directed)
+  DeviceCsr A;
+  A.n = n;
+  A.indptr = DevBuf((n + 1) * 8);
+  DevBuf cnt(std::max<uint64_t>(n, 1) * 4);
+  const uint32_t R2 = rows_per_block(k, 2);
+  const size_t lds2 = (size_t)R2 * (k | 1u) * 8;
+  if (n) {
+    memb_count_kernel<<<grid_for(n, R2), R2, lds2, s>>>(idx, dists, n, k, p.disconnection_distance,
+                                                         out->sigmas.as<float>(), out->rhos.as<float>(),
+                                                         cnt.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
+  exclusive_scan(cnt.as<uint32_t>(), A.indptr.as<uint64_t>(), n, s);
+  UMAP_HIP(hipMemcpyAsync(&A.nnz, A.indptr.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  const uint64_t E = A.nnz;
+  out->nnz_directed = E;
+  A.indices = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  A.data = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  DevBuf rows;
+  if (p.symmetrize) rows = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  if (n) {
+    memb_fill_kernel<<<grid_for(n, R2), R2, lds2, s>>>(idx, dists, n, k, p.disconnection_distance,
+                                                        out->sigmas.as<float>(), out->rhos.as<float>(),
+                                                        A.indptr.as<uint64_t>(), A.indices.as<uint32_t>(),
+                                                        A.data.as<float>(), p.symmetrize ? rows.as<uint32_t>() : nullptr);
+    UMAP_LAUNCH_CHECK();
+  }
+  timer.mark("build_membership_csr");
+
+  if (!p.symmetrize) {
+    out->graph = std::move(A);
+  } else {
+    // transpose: stable sort of (col, row, w) by col => Aᵀ in CSR order
+    DevBuf tkeys(std::max<uint64_t>(E, 1) * 4), tw(std::max<uint64_t>(E, 1) * 4);
+    if (E) {
+      UMAP_HIP(hipMemcpyAsync(tkeys.p, A.indices.p, E * 4, hipMemcpyDeviceToDevice, s));
+      UMAP_HIP(hipMemcpyAsync(tw.p, A.data.p, E * 4, hipMemcpyDeviceToDevice, s));
+    }
+    radix_sort_pairs(tkeys.as<uint32_t>(), rows.as<uint32_t>(), tw.as<uint32_t>(), E,
+                     n ? (uint32_t)(n - 1) : 0u, s);
+    DevBuf tptr((n + 1) * 8);
+    if (E == 0) {
+      UMAP_HIP(hipMemsetAsync(tptr.p, 0, (n + 1) * 8, s));
+    } else {
+      transpose_ptr_kernel<<<grid_for(E, 256), 256, 0, s>>>(tkeys.as<uint32_t>(), E, n, tptr.as<uint64_t>());
+      UMAP_LAUNCH_CHECK();
+    }
+    timer.mark("set_operations transpose");
+    const float mix = p.set_op_mix_ratio;
+    const float prod = 1.0f - 2.0f * mix;  // fuzzy_simplicial_set.rs:394
+    DeviceCsr S;
+    S.n = n;
+    S.indptr = DevBuf((n + 1) * 8);
+    if (n) {
+      merge_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(
+          A.indptr.as<uint64_t>(), A.indices.as<uint32_t>(), A.data.as<float>(), tptr.as<uint64_t>(),
+          rows.as<uint32_t>(), tw.as<float>(), n, mix, prod, cnt.as<uint32_t>(), nullptr, nullptr, nullptr);
+      UMAP_LAUNCH_CHECK();
+    }
+    exclusive_scan(cnt.as<uint32_t>(), S.indptr.as<uint64_t>(), n, s);
+    UMAP_HIP(hipMemcpyAsync(&S.nnz, S.indptr.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    S.indices = DevBuf(std::max<uint64_t>(S.nnz, 1) * 4);
+    S.data = DevBuf(std::max<uint64_t>(S.nnz, 1) * 4);
+    if (n) {
+      merge_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(
+          A.indptr.as<uint64_t>(), A.indices.as<uint32_t>(), A.data.as<float>(), tptr.as<uint64_t>(),
+          rows.as<uint32_t>(), tw.as<float>(), n, mix, prod, nullptr, S.indptr.as<uint64_t>(),
+          S.indices.as<uint32_t>(), S.data.as<float>());
+      UMAP_LAUNCH_CHECK();
+    }
+    UMAP_HIP(hipStreamSynchronize(s));
+    out->graph = std::move(S);
+    timer.mark("set_operations merge");
+  }
+
+  device_zero_rows(out->graph, &out->zero_rows, &out->vertices_disconnected, s);
+  timer.mark("disconnected vertices");
+  UMAP_HIP(hipMemcpyAsync(&out->edges_removed, ctr.p, 8, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  timer.collect(&out->timings);
+}
+
+void device_zero_rows(const DeviceCsr& g, DevBuf* flags, uint64_t* count, hipStream_t s) {
+  *flags = DevBuf(std::max<uint64_t>(g.n, 1));
+  DevBuf ctr(8);
+  UMAP_HIP(hipMemsetAsync(ctr.p, 0, 8, s));
+  if (g.n) {
+    zero_rows_kernel<<<grid_for(g.n, 256), 256, 0, s>>>(g.indptr.as<uint64_t>(), g.data.as<float>(), g.n,
+                                                         flags->as<uint8_t>(), ctr.as<unsigned long long>());
+    UMAP_LAUNCH_CHECK();
+  }
+  UMAP_HIP(hipMemcpyAsync(count, ctr.p, 8, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void device_copy_calibration(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n16 = bytes / 16;
+  if (!n16) return;
+  copy_calibration_kernel<<<(unsigned)std::min<uint64_t>(ceil_div(n16, 256), 1u << 16), 256, 0, s>>>(
+      static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  UMAP_LAUNCH_CHECK();
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void device_expf(const float* x, float* y, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  expf_kernel<<<(unsigned)std::min<uint64_t>(grid_for(n, 256), 8192), 256, 0, s>>>(x, y, n);
+  UMAP_LAUNCH_CHECK();
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace umapgpu

@@ -1,0 +1,57 @@
+// Graph construction ("learn_manifold") on the device.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace umapgpu {
+
+struct GraphParams {
+  uint64_t n = 0;        // samples (rows of the kNN arrays)
+  uint32_t k = 0;        // kNN width (columns)
+  float target = 0.0f;   // log2(n_neighbors) * bandwidth (host glibc log2f)
+  int32_t n_iter = 64;   // bisection cap (smooth_knn_dist.rs:12-13)
+  float local_connectivity = 1.0f;
+  float disconnection_distance = INFINITY;
+  float set_op_mix_ratio = 1.0f;
+  bool symmetrize = true;
+  bool contiguous_input = true;  // caller's dists array was one contiguous slice
+};
+
+struct DeviceCsr {
+  uint64_t n = 0, nnz = 0;
+  DevBuf indptr;   // u64[n+1]
+  DevBuf indices;  // u32[nnz]
+  DevBuf data;     // f32[nnz]
+};
+
+struct StageTime {
+  std::string name;
+  double ms;
+};
+
+struct GraphResult {
+  DeviceCsr graph;
+  DevBuf sigmas, rhos;          // f32[n]
+  DevBuf zero_rows;             // u8[n]: row weights sum to exactly 0
+  uint64_t edges_removed = 0;   // kNN slots with dist >= disconnection_distance
+  uint64_t vertices_disconnected = 0;
+  uint64_t nnz_directed = 0;
+  bool used_global_mean = false;
+  std::vector<StageTime> timings;
+};
+
+// idx/dists: device arrays, row-major n x k, contiguous.
+void build_graph(const uint32_t* idx, const float* dists, const GraphParams& p, GraphResult* out, hipStream_t s);
+
+// Device-side copy helpers used by the API layer.
+void device_zero_rows(const DeviceCsr& g, DevBuf* flags, uint64_t* count, hipStream_t s);
+
+// Streaming copy of bytes/16 uint4 words (traffic-counter calibration).
+void device_copy_calibration(const void* src, void* dst, uint64_t bytes, hipStream_t s);
+
+// Test hook: exp of every x[i] through the device glibc-equivalent expf.
+void device_expf(const float* x, float* y, uint64_t n, hipStream_t s);
+
+}  // namespace umapgpu

@@ -1,0 +1,659 @@
+// Optimizer::new preprocessing and the Euclidean SGD epoch on the device.
+//
+// Reference stages (umap-rs 0.4.5):
+//   edge threshold filter      optimizer.rs:88-167
+//   make_epochs_per_sample     umap/make_epochs_per_sample.rs:21-52
+//   [0,10] normalisation       optimizer.rs:181-234
+//   schedule arrays            optimizer.rs:236-272
+//   step_epochs                optimizer.rs:297-366
+//   Euclidean SGD epoch        layout/optimize_layout_euclidean.rs:236-347
+//
+// SGD design.  The reference's Rayon loop hands each thread a contiguous
+// range of edges; since edges are in CSR row order, one thread walks a head
+// row's edges in order and H[j] only ever changes under that thread.  Here
+// one lane owns one head row for the whole epoch: H[j] lives in registers
+// (read once, written once), the lane walks the row's edges in order, and
+// the positive and negative samples run through one flattened loop so that
+// the 64 lanes of a wave stay busy whatever their rows' activity pattern.
+// Rows are dealt to lanes by descending degree so a wave's lanes finish
+// together.  The tail snapshot is shared by all lanes: its attractive
+// updates are f32 hardware atomics (or plain racy stores, as the reference).
+// The f64 schedule recurrence is evaluated exactly as the reference does, so
+// every positive update and every negative-sample count is identical.
+#include <hip/hip_runtime.h>
+
+#include "optimize.hpp"
+#include "primitives.hpp"
+
+namespace umapgpu {
+
+namespace {
+
+unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)ceil_div(n, block); }
+
+__global__ void filter_count_kernel(const uint64_t* __restrict__ ptr, const float* __restrict__ data, uint64_t n,
+                                    float thr, uint32_t* __restrict__ cnt) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t c = 0;
+  for (uint64_t e = ptr[r]; e < ptr[r + 1]; ++e) c += (data[e] >= thr) ? 1u : 0u;
+  cnt[r] = c;
+}
+
+__global__ void filter_fill_kernel(const uint64_t* __restrict__ ptr, const uint32_t* __restrict__ ind,
+                                   const float* __restrict__ data, uint64_t n, float thr,
+                                   const uint64_t* __restrict__ optr, uint32_t* __restrict__ tail,
+                                   float* __restrict__ w) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint64_t o = optr[r];
+  for (uint64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
+    const float v = data[e];
+    if (v >= thr) {
+      tail[o] = ind[e];
+      w[o] = v;
+      ++o;
+    }
+  }
+}
+
+template <typename NextT>
+__device__ inline NextT next_epoch(double eons) {
+  // smallest integer epoch e >= 0 with eons <= e; MAX means "never" (the
+  // caller guarantees every epoch index is below MAX)
+  const NextT mx = (NextT)~(NextT)0;
+  if (!(eons <= (double)(mx - 1))) return mx;
+  if (eons <= 0.0) return (NextT)0;
+  return (NextT)ceil(eons);
+}
+
+// make_epochs_per_sample + the three schedule arrays, fused.
+template <typename NextT>
+__global__ void schedule_init_kernel(const float* __restrict__ w, uint64_t e_count, double n_epochs, double max_w,
+                                     double neg_rate, double2* __restrict__ rate, double2* __restrict__ sched,
+                                     NextT* __restrict__ next) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= e_count) return;
+  const double ns = n_epochs * ((double)w[i] / max_w);
+  const double eps = (ns > 0.0) ? n_epochs / ns : -1.0;
+  const double epns = eps / neg_rate;
+  rate[i] = make_double2(eps, epns);
+  sched[i] = make_double2(eps, epns);
+  next[i] = next_epoch<NextT>(eps);
+}
+
+template <typename NextT>
+__global__ void next_from_sched_kernel(const double2* __restrict__ sched, uint64_t e_count, NextT* __restrict__ next) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < e_count) next[i] = next_epoch<NextT>(sched[i].x);
+}
+
+__global__ void normalize_kernel(float* __restrict__ emb, uint64_t count, uint32_t dim, const float* __restrict__ mins,
+                                 const float* __restrict__ scales) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = (uint32_t)(i % dim);
+    const float sc = scales[d];
+    if (sc > 0.0f) emb[i] = (emb[i] - mins[d]) * sc;
+  }
+}
+
+__global__ void degree_keys_kernel(const uint64_t* __restrict__ ptr, uint64_t n, uint32_t max_deg,
+                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ rows) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t deg = ptr[r + 1] - ptr[r];
+  keys[r] = max_deg - (uint32_t)deg;
+  rows[r] = (uint32_t)r;
+}
+
+__global__ void max_degree_kernel(const uint64_t* __restrict__ ptr, uint64_t n, unsigned int* __restrict__ out) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int d = 0;
+  if (r < n) {
+    const uint64_t deg = ptr[r + 1] - ptr[r];
+    d = deg > 0xffffffffull ? 0xffffffffu : (unsigned int)deg;
+  }
+  for (int off = 32; off > 0; off >>= 1) d = max(d, (unsigned int)__shfl_xor((int)d, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, d);
+}
+
+struct SgdArgs {
+  float a, b;
+  float m2ab;   // (-2*a)*b   (optimize_layout_euclidean.rs:292)
+  float g2b;    // (2*gamma)*b (:329)
+  float alpha;  // lr * (1 - epoch/total) (optimizer.rs:317)
+  uint32_t n;   // n_vertices
+  uint64_t epoch;
+  double epoch_f64;
+  uint64_t epoch_key;
+};
+
+__device__ inline void wave_add_stats(uint64_t npos, uint64_t nneg, unsigned long long* stats) {
+  for (int off = 32; off > 0; off >>= 1) {
+    npos += __shfl_xor(npos, off, 64);
+    nneg += __shfl_xor(nneg, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (npos | nneg)) {
+    atomicAdd(&stats[0], (unsigned long long)npos);
+    atomicAdd(&stats[1], (unsigned long long)nneg);
+  }
+}
+
+// Attractive coefficient (optimize_layout_euclidean.rs:290-297).
+__device__ inline float attract_coeff(const SgdArgs& A, float d2, float pb) {
+  float gc = (A.m2ab * pb) / d2;
+  gc = gc / ((A.a * pb) * d2 + 1.0f);
+  return gc;
+}
+// Repulsive coefficient (:327-332).
+__device__ inline float repel_coeff(const SgdArgs& A, float d2, float pb) {
+  return A.g2b / ((0.001f + d2) * ((A.a * pb) * d2 + 1.0f));
+}
+
+// One lane = one head row.  D = embedding dimension (registers).
+template <int D, typename NextT, bool ATOMIC>
+__global__ __launch_bounds__(256) void sgd_rows_kernel(float* __restrict__ H, float* __restrict__ T,
+                                                       const uint64_t* __restrict__ row_ptr,
+                                                       const uint32_t* __restrict__ tail,
+                                                       const double2* __restrict__ rate, double2* __restrict__ sched,
+                                                       NextT* __restrict__ next, const uint32_t* __restrict__ perm,
+                                                       SgdArgs A, unsigned long long* __restrict__ stats) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t npos = 0, nneg = 0;
+  if (gid < A.n) {
+    const uint32_t j = perm[gid];
+    uint64_t e = row_ptr[j];
+    const uint64_t e1 = row_ptr[j + 1];
+    float h[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) h[d] = H[(uint64_t)j * D + d];
+    const NextT ep = (NextT)A.epoch;
+    uint32_t neg_left = 0, p = 0;
+    uint64_t ekey = 0;
+    for (;;) {
+      uint32_t kk;
+      bool pos;
+      if (neg_left == 0) {
+        while (e < e1 && next[e] > ep) ++e;
+        if (e >= e1) break;
+        pos = true;
+        kk = tail[e];
+      } else {
+        kk = rng_index(ekey, p, A.n);
+        ++p;
+        --neg_left;
+        pos = false;
+        if (kk == j) continue;  // the draw is consumed (:315-317)
+      }
+      float tv[D], diff[D];
+      float d2 = 0.0f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        tv[d] = T[(uint64_t)kk * D + d];
+        diff[d] = h[d] - tv[d];
+        d2 = d2 + diff[d] * diff[d];
+      }
+      float gc = 0.0f;
+      if (d2 > 0.0f) {
+        const float pb = powf(d2, A.b);
+        gc = pos ? attract_coeff(A, d2, pb) : repel_coeff(A, d2, pb);
+      }
+      if (pos || gc > 0.0f) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float ga = clip4(gc * diff[d]) * A.alpha;
+          h[d] = h[d] + ga;
+          if (pos) {
+            float* tp = T + (uint64_t)kk * D + d;
+            if (ATOMIC)
+              __hip_atomic_fetch_add(tp, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+              *tp = tv[d] - ga;
+          }
+        }
+      }
+      if (pos) {
+        double2 sc = sched[e];
+        const double2 rt = rate[e];
+        sc.x = sc.x + rt.x;  // epoch_of_next_sample += epochs_per_sample (:308)
+        const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // (:310-311)
+        sc.y = sc.y + (double)nn * rt.y;  // (:343)
+        sched[e] = sc;
+        next[e] = next_epoch<NextT>(sc.x);
+        ekey = rng_edge_key(A.epoch_key, e);
+        p = 0;
+        neg_left = nn;
+        ++e;
+        ++npos;
+        nneg += nn;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) H[(uint64_t)j * D + d] = h[d];
+  }
+  wave_add_stats(npos, nneg, stats);
+}
+
+// Any dimension: the head row stays in global memory (only this lane
+// touches it).
+template <typename NextT, bool ATOMIC>
+__global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H, float* __restrict__ T, uint32_t dim,
+                                                           const uint64_t* __restrict__ row_ptr,
+                                                           const uint32_t* __restrict__ tail,
+                                                           const double2* __restrict__ rate,
+                                                           double2* __restrict__ sched, NextT* __restrict__ next,
+                                                           const uint32_t* __restrict__ perm, SgdArgs A,
+                                                           unsigned long long* __restrict__ stats) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t npos = 0, nneg = 0;
+  if (gid < A.n) {
+    const uint32_t j = perm[gid];
+    float* h = H + (uint64_t)j * dim;
+    const NextT ep = (NextT)A.epoch;
+    for (uint64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
+      if (next[e] > ep) continue;
+      const uint32_t k = tail[e];
+      float* t = T + (uint64_t)k * dim;
+      float d2 = 0.0f;
+      for (uint32_t d = 0; d < dim; ++d) {
+        const float df = h[d] - t[d];
+        d2 = d2 + df * df;
+      }
+      float gc = 0.0f;
+      if (d2 > 0.0f) gc = attract_coeff(A, d2, powf(d2, A.b));
+      for (uint32_t d = 0; d < dim; ++d) {
+        const float tv = t[d];
+        const float ga = clip4(gc * (h[d] - tv)) * A.alpha;
+        h[d] = h[d] + ga;
+        if (ATOMIC)
+          __hip_atomic_fetch_add(t + d, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          t[d] = tv - ga;
+      }
+      double2 sc = sched[e];
+      const double2 rt = rate[e];
+      sc.x = sc.x + rt.x;
+      const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);
+      const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+      for (uint32_t p = 0; p < nn; ++p) {
+        const uint32_t kk = rng_index(ekey, p, A.n);
+        if (kk == j) continue;
+        const float* o = T + (uint64_t)kk * dim;
+        float dd = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+          const float df = h[d] - o[d];
+          dd = dd + df * df;
+        }
+        float g2 = 0.0f;
+        if (dd > 0.0f) g2 = repel_coeff(A, dd, powf(dd, A.b));
+        if (g2 > 0.0f)
+          for (uint32_t d = 0; d < dim; ++d) h[d] = h[d] + clip4(g2 * (h[d] - o[d])) * A.alpha;
+      }
+      sc.y = sc.y + (double)nn * rt.y;
+      sched[e] = sc;
+      next[e] = next_epoch<NextT>(sc.x);
+      ++npos;
+      nneg += nn;
+    }
+  }
+  wave_add_stats(npos, nneg, stats);
+}
+
+// Deterministic serial mode: the reference's edge order on one lane
+// (optimize_layout_euclidean.rs:168-231 executed in order).
+template <typename NextT>
+__global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, uint32_t dim,
+                                  const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ tail,
+                                  const double2* __restrict__ rate, double2* __restrict__ sched,
+                                  NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t npos = 0, nneg = 0;
+  for (uint32_t j = 0; j < A.n; ++j) {
+    float* h = H + (uint64_t)j * dim;
+    for (uint64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
+      double2 sc = sched[e];
+      if (!(sc.x <= A.epoch_f64)) continue;
+      const uint32_t k = tail[e];
+      float* t = T + (uint64_t)k * dim;
+      float d2 = 0.0f;
+      for (uint32_t d = 0; d < dim; ++d) {
+        const float df = h[d] - t[d];
+        d2 = d2 + df * df;
+      }
+      float gc = 0.0f;
+      if (d2 > 0.0f) gc = attract_coeff(A, d2, powf(d2, A.b));
+      for (uint32_t d = 0; d < dim; ++d) {
+        const float df = h[d] - t[d];
+        const float g = clip4(gc * df);
+        h[d] = h[d] + g * A.alpha;
+        t[d] = t[d] - g * A.alpha;
+      }
+      const double2 rt = rate[e];
+      sc.x = sc.x + rt.x;
+      const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);
+      const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+      for (uint32_t p = 0; p < nn; ++p) {
+        const uint32_t kk = rng_index(ekey, p, A.n);
+        if (kk == j) continue;
+        const float* o = T + (uint64_t)kk * dim;
+        float dd = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+          const float df = h[d] - o[d];
+          dd = dd + df * df;
+        }
+        float g2 = 0.0f;
+        if (dd > 0.0f) g2 = repel_coeff(A, dd, powf(dd, A.b));
+        if (g2 > 0.0f)
+          for (uint32_t d = 0; d < dim; ++d) {
+            const float df = h[d] - o[d];
+            h[d] = h[d] + clip4(g2 * df) * A.alpha;
+          }
+      }
+      sc.y = sc.y + (double)nn * rt.y;
+      sched[e] = sc;
+      next[e] = next_epoch<NextT>(sc.x);
+      ++npos;
+      nneg += nn;
+    }
+  }
+  stats[0] += npos;
+  stats[1] += nneg;
+}
+
+template <typename NextT, bool ATOMIC>
+void launch_rows(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  const unsigned grid = grid_for(st->n, 256);
+  float* H = st->head_emb.as<float>();
+  float* T = st->tail_emb.as<float>();
+  const uint64_t* rp = st->row_ptr.as<uint64_t>();
+  const uint32_t* tl = st->tail.as<uint32_t>();
+  const double2* rt = st->rate.as<double2>();
+  double2* sc = st->sched.as<double2>();
+  NextT* nx = st->next.as<NextT>();
+  const uint32_t* pm = st->perm.as<uint32_t>();
+  unsigned long long* stt = st->stats.as<unsigned long long>();
+#define UMAP_ROWS_CASE(DD)                                                                       \
+  case DD:                                                                                       \
+    sgd_rows_kernel<DD, NextT, ATOMIC><<<grid, 256, 0, s>>>(H, T, rp, tl, rt, sc, nx, pm, A, stt); \
+    break;
+  switch (st->dim) {
+    UMAP_ROWS_CASE(1)
+    UMAP_ROWS_CASE(2)
+    UMAP_ROWS_CASE(3)
+    UMAP_ROWS_CASE(4)
+    UMAP_ROWS_CASE(5)
+    UMAP_ROWS_CASE(6)
+    UMAP_ROWS_CASE(7)
+    UMAP_ROWS_CASE(8)
+    default:
+      sgd_rows_dyn_kernel<NextT, ATOMIC><<<grid, 256, 0, s>>>(H, T, (uint32_t)st->dim, rp, tl, rt, sc, nx, pm, A, stt);
+  }
+#undef UMAP_ROWS_CASE
+  UMAP_LAUNCH_CHECK();
+}
+
+template <typename NextT>
+void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  switch (st->mode) {
+    case SgdMode::Serial:
+      sgd_serial_kernel<NextT><<<1, 64, 0, s>>>(st->head_emb.as<float>(), st->tail_emb.as<float>(),
+                                                (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
+                                                st->tail.as<uint32_t>(), st->rate.as<double2>(),
+                                                st->sched.as<double2>(), st->next.as<NextT>(), A,
+                                                st->stats.as<unsigned long long>());
+      UMAP_LAUNCH_CHECK();
+      break;
+    case SgdMode::HogwildRacy:
+      launch_rows<NextT, false>(st, A, s);
+      break;
+    default:
+      launch_rows<NextT, true>(st, A, s);
+  }
+}
+
+void build_perm(OptimizerState* st, hipStream_t s) {
+  const uint64_t n = st->n;
+  st->perm = DevBuf(std::max<uint64_t>(n, 1) * 4);
+  if (n == 0) return;
+  DevBuf mx(4);
+  UMAP_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+  max_degree_kernel<<<grid_for(n, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), n, mx.as<unsigned int>());
+  UMAP_LAUNCH_CHECK();
+  uint32_t max_deg = 0;
+  UMAP_HIP(hipMemcpyAsync(&max_deg, mx.p, 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  DevBuf keys(n * 4), dummy(n * 4);
+  degree_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), n, max_deg, keys.as<uint32_t>(),
+                                                       st->perm.as<uint32_t>());
+  UMAP_LAUNCH_CHECK();
+  radix_sort_pairs(keys.as<uint32_t>(), st->perm.as<uint32_t>(), dummy.as<uint32_t>(), n, max_deg, s);
+}
+
+void alloc_next(OptimizerState* st) {
+  st->next16 = st->total_epochs <= 65535;
+  st->next = DevBuf(std::max<uint64_t>(st->n_edges, 1) * (st->next16 ? 2 : 4));
+}
+
+}  // namespace
+
+void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t dim, const PrepareArgs& args,
+                       OptimizerState* st, hipStream_t s) {
+  const uint64_t n = g.n;
+  hipEvent_t ev0, ev1;
+  UMAP_HIP(hipEventCreate(&ev0));
+  UMAP_HIP(hipEventCreate(&ev1));
+  UMAP_HIP(hipEventRecord(ev0, s));
+  st->n = n;
+  st->dim = dim;
+  st->total_epochs = args.total_epochs;
+  st->current_epoch = 0;
+  st->a = args.a;
+  st->b = args.b;
+  st->gamma = args.repulsion_strength;
+  st->initial_alpha = args.learning_rate;
+  st->negative_sample_rate = (double)args.negative_sample_rate;
+  st->metric_type = args.metric_type;
+  st->seed = args.seed;
+  if (args.total_epochs > 0xffffffffull) throw Error(Err::Unsupported, "total_epochs above 2^32-1 is not supported");
+
+  // threshold (optimizer.rs:90-102)
+  const float max_val = reduce_max_f32(g.data.as<float>(), g.nnz, 1, 0.0f, s);
+  const uint64_t default_epochs = (n <= 10000) ? 500 : 200;
+  const uint64_t threshold_epochs = (args.total_epochs > 10) ? args.total_epochs : default_epochs;
+  const float threshold = max_val / (float)threshold_epochs;
+
+  // filter + extract (:108-167)
+  DevBuf cnt(std::max<uint64_t>(n, 1) * 4);
+  st->row_ptr = DevBuf((n + 1) * 8);
+  if (n) {
+    filter_count_kernel<<<grid_for(n, 256), 256, 0, s>>>(g.indptr.as<uint64_t>(), g.data.as<float>(), n, threshold,
+                                                          cnt.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
+  exclusive_scan(cnt.as<uint32_t>(), st->row_ptr.as<uint64_t>(), n, s);
+  UMAP_HIP(hipMemcpyAsync(&st->n_edges, st->row_ptr.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  const uint64_t E = st->n_edges;
+  st->tail = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  DevBuf w(std::max<uint64_t>(E, 1) * 4);
+  if (n) {
+    filter_fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(g.indptr.as<uint64_t>(), g.indices.as<uint32_t>(),
+                                                         g.data.as<float>(), n, threshold, st->row_ptr.as<uint64_t>(),
+                                                         st->tail.as<uint32_t>(), w.as<float>());
+    UMAP_LAUNCH_CHECK();
+  }
+
+  // make_epochs_per_sample + schedule arrays
+  float max_w = reduce_max_f32(w.as<float>(), E, 1, -INFINITY, s);
+  if (max_w <= 0.0f) max_w = 1.0f;
+  st->rate = DevBuf(std::max<uint64_t>(E, 1) * 16);
+  st->sched = DevBuf(std::max<uint64_t>(E, 1) * 16);
+  alloc_next(st);
+  if (E) {
+    if (st->next16)
+      schedule_init_kernel<uint16_t><<<grid_for(E, 256), 256, 0, s>>>(
+          w.as<float>(), E, (double)args.total_epochs, (double)max_w, st->negative_sample_rate,
+          st->rate.as<double2>(), st->sched.as<double2>(), st->next.as<uint16_t>());
+    else
+      schedule_init_kernel<uint32_t><<<grid_for(E, 256), 256, 0, s>>>(
+          w.as<float>(), E, (double)args.total_epochs, (double)max_w, st->negative_sample_rate,
+          st->rate.as<double2>(), st->sched.as<double2>(), st->next.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
+
+  // embedding normalisation to [0, 10] per dimension (:184-230)
+  st->head_emb = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
+  st->tail_emb = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
+  if (n * dim) {
+    UMAP_HIP(hipMemcpyAsync(st->head_emb.p, init_device, n * dim * 4, hipMemcpyDeviceToDevice, s));
+    std::vector<float> mins(dim), scales(dim);
+    for (uint64_t d = 0; d < dim; ++d) {
+      const float mn = reduce_min_f32(st->head_emb.as<float>() + d, n, dim, INFINITY, s);
+      const float mxv = reduce_max_f32(st->head_emb.as<float>() + d, n, dim, -INFINITY, s);
+      const float range = mxv - mn;
+      mins[d] = mn;
+      scales[d] = (range > 0.0f) ? 10.0f / range : 0.0f;
+    }
+    DevBuf dmins(dim * 4), dscales(dim * 4);
+    UMAP_HIP(hipMemcpyAsync(dmins.p, mins.data(), dim * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(dscales.p, scales.data(), dim * 4, hipMemcpyHostToDevice, s));
+    normalize_kernel<<<(unsigned)std::min<uint64_t>(grid_for(n * dim, 256), 16384), 256, 0, s>>>(
+        st->head_emb.as<float>(), n * dim, (uint32_t)dim, dmins.as<float>(), dscales.as<float>());
+    UMAP_LAUNCH_CHECK();
+    UMAP_HIP(hipStreamSynchronize(s));
+  }
+  build_perm(st, s);
+  st->stats = DevBuf(16);
+  UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
+  UMAP_HIP(hipEventRecord(ev1, s));
+  UMAP_HIP(hipEventSynchronize(ev1));
+  float ms = 0;
+  UMAP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  st->prep_timings.push_back({"optimizer new", (double)ms});
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+}
+
+void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t* head, const uint32_t* tail,
+                      const double* eps, const double* eons, const double* epns, const double* eonns,
+                      const float* embedding, OptimizerState* st, hipStream_t s) {
+  st->n = n;
+  st->dim = dim;
+  st->n_edges = n_edges;
+  std::vector<uint64_t> rp(n + 1, 0);
+  for (uint64_t e = 0; e < n_edges; ++e) {
+    if (head[e] >= n) throw Error(Err::InvalidArgument, "checkpoint head index out of range");
+    if (e && head[e] < head[e - 1]) throw Error(Err::InvalidArgument, "checkpoint head array must be non-decreasing");
+    if (tail[e] >= n) throw Error(Err::InvalidArgument, "checkpoint tail index out of range");
+    rp[head[e] + 1]++;
+  }
+  for (uint64_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+  std::vector<double> r2(2 * std::max<uint64_t>(n_edges, 1)), s2(2 * std::max<uint64_t>(n_edges, 1));
+  for (uint64_t e = 0; e < n_edges; ++e) {
+    r2[2 * e] = eps[e];
+    r2[2 * e + 1] = epns[e];
+    s2[2 * e] = eons[e];
+    s2[2 * e + 1] = eonns[e];
+  }
+  const uint64_t E1 = std::max<uint64_t>(n_edges, 1);
+  st->row_ptr = DevBuf((n + 1) * 8);
+  st->tail = DevBuf(E1 * 4);
+  st->rate = DevBuf(E1 * 16);
+  st->sched = DevBuf(E1 * 16);
+  alloc_next(st);
+  st->head_emb = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
+  st->tail_emb = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
+  UMAP_HIP(hipMemcpyAsync(st->row_ptr.p, rp.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  if (n_edges) {
+    UMAP_HIP(hipMemcpyAsync(st->tail.p, tail, n_edges * 4, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(st->rate.p, r2.data(), n_edges * 16, hipMemcpyHostToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(st->sched.p, s2.data(), n_edges * 16, hipMemcpyHostToDevice, s));
+    if (st->next16)
+      next_from_sched_kernel<uint16_t><<<grid_for(n_edges, 256), 256, 0, s>>>(st->sched.as<double2>(), n_edges,
+                                                                              st->next.as<uint16_t>());
+    else
+      next_from_sched_kernel<uint32_t><<<grid_for(n_edges, 256), 256, 0, s>>>(st->sched.as<double2>(), n_edges,
+                                                                              st->next.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
+  if (n * dim) UMAP_HIP(hipMemcpyAsync(st->head_emb.p, embedding, n * dim * 4, hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  build_perm(st, s);
+  st->stats = DevBuf(16);
+  UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
+  const uint64_t start = st->current_epoch, end = st->current_epoch + n_epochs;
+  if (st->n * st->dim)
+    UMAP_HIP(hipMemcpyAsync(st->tail_emb.p, st->head_emb.p, st->n * st->dim * 4, hipMemcpyDeviceToDevice, s));
+  hipEvent_t ev0, ev1;
+  UMAP_HIP(hipEventCreate(&ev0));
+  UMAP_HIP(hipEventCreate(&ev1));
+  UMAP_HIP(hipEventRecord(ev0, s));
+  if (st->n && st->n_edges) {
+    SgdArgs A;
+    A.a = st->a;
+    A.b = st->b;
+    A.m2ab = (-2.0f * st->a) * st->b;
+    A.g2b = (2.0f * st->gamma) * st->b;
+    A.n = (uint32_t)st->n;
+    for (uint64_t e = start; e < end; ++e) {
+      A.alpha = st->initial_alpha * (1.0f - ((float)e / (float)st->total_epochs));
+      A.epoch = e;
+      A.epoch_f64 = (double)e;
+      A.epoch_key = rng_epoch_key(st->seed, e);
+      if (st->next16)
+        launch_epoch<uint16_t>(st, A, s);
+      else
+        launch_epoch<uint32_t>(st, A, s);
+    }
+  }
+  UMAP_HIP(hipEventRecord(ev1, s));
+  UMAP_HIP(hipEventSynchronize(ev1));
+  float ms = 0;
+  UMAP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  st->last_step_ms = ms;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  st->current_epoch = end;
+}
+
+void optimizer_export(const OptimizerState& st, uint32_t* head, uint32_t* tail, double* eps, double* eons,
+                      double* epns, double* eonns, float* embedding, hipStream_t s) {
+  const uint64_t E = st.n_edges;
+  if (head) {
+    std::vector<uint64_t> rp(st.n + 1);
+    UMAP_HIP(hipMemcpyAsync(rp.data(), st.row_ptr.p, (st.n + 1) * 8, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    for (uint64_t j = 0; j < st.n; ++j)
+      for (uint64_t e = rp[j]; e < rp[j + 1]; ++e) head[e] = (uint32_t)j;
+  }
+  if (tail && E) UMAP_HIP(hipMemcpyAsync(tail, st.tail.p, E * 4, hipMemcpyDeviceToHost, s));
+  if ((eps || epns || eons || eonns) && E) {
+    std::vector<double> r2(2 * E), s2(2 * E);
+    UMAP_HIP(hipMemcpyAsync(r2.data(), st.rate.p, E * 16, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipMemcpyAsync(s2.data(), st.sched.p, E * 16, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    for (uint64_t e = 0; e < E; ++e) {
+      if (eps) eps[e] = r2[2 * e];
+      if (epns) epns[e] = r2[2 * e + 1];
+      if (eons) eons[e] = s2[2 * e];
+      if (eonns) eonns[e] = s2[2 * e + 1];
+    }
+  }
+  if (embedding && st.n * st.dim)
+    UMAP_HIP(hipMemcpyAsync(embedding, st.head_emb.p, st.n * st.dim * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void optimizer_read_stats(const OptimizerState& st, uint64_t* pos, uint64_t* neg, hipStream_t s) {
+  uint64_t v[2] = {0, 0};
+  UMAP_HIP(hipMemcpyAsync(v, st.stats.p, 16, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  if (pos) *pos = v[0];
+  if (neg) *neg = v[1];
+}
+
+}  // namespace umapgpu

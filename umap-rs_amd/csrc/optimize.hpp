@@ -1,0 +1,63 @@
+// Optimizer state and SGD on the device.
+#pragma once
+#include "graph.hpp"
+
+namespace umapgpu {
+
+enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2 };
+
+// Device-resident Optimizer (optimizer.rs:28-56).  Edges are kept in the
+// reference's CSR row order, so an edge's head is its row in `row_ptr`.
+struct OptimizerState {
+  uint64_t n = 0, dim = 0, n_edges = 0;
+  uint64_t total_epochs = 0, current_epoch = 0;
+  float a = 0, b = 0, gamma = 1, initial_alpha = 1;
+  double negative_sample_rate = 5.0;
+  int32_t metric_type = 0;
+  uint64_t seed = 0;
+  SgdMode mode = SgdMode::Hogwild;
+  bool next16 = true;  // next-epoch counters fit in u16 (total_epochs <= 65535)
+
+  DevBuf head_emb;  // f32[n*dim]: Optimizer::embedding
+  DevBuf tail_emb;  // f32[n*dim]: per-call snapshot (optimizer.rs:314)
+  DevBuf row_ptr;   // u64[n+1]
+  DevBuf tail;      // u32[E]
+  DevBuf rate;      // double2[E]: {epochs_per_sample, epochs_per_negative_sample}
+  DevBuf sched;     // double2[E]: {epoch_of_next_sample, epoch_of_next_negative_sample}
+  DevBuf next;      // u16/u32[E]: ceil(epoch_of_next_sample), the activity test
+  DevBuf perm;      // u32[n]: rows by descending degree (lane load balance)
+  DevBuf stats;     // u64[2]: positive updates, negative samples
+
+  double last_step_ms = 0.0;
+  std::vector<StageTime> prep_timings;
+};
+
+struct PrepareArgs {
+  uint64_t total_epochs;
+  uint64_t negative_sample_rate;
+  float repulsion_strength, learning_rate;
+  float a, b;
+  int32_t metric_type;
+  uint64_t seed;
+};
+
+// Optimizer::new (optimizer.rs:74-290) from a device CSR and a device init
+// (n x dim, copied).
+void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t dim, const PrepareArgs& args,
+                       OptimizerState* st, hipStream_t s);
+
+// Rebuild from exported state (edge arrays on the host).
+void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t* head, const uint32_t* tail,
+                      const double* eps, const double* eons, const double* epns, const double* eonns,
+                      const float* embedding, OptimizerState* st, hipStream_t s);
+
+// Euclidean step_epochs body (optimizer.rs:306-365).  Overstep is checked by
+// the caller.
+void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s);
+
+void optimizer_export(const OptimizerState& st, uint32_t* head, uint32_t* tail, double* eps, double* eons,
+                      double* epns, double* eonns, float* embedding, hipStream_t s);
+
+void optimizer_read_stats(const OptimizerState& st, uint64_t* pos, uint64_t* neg, hipStream_t s);
+
+}  // namespace umapgpu

@@ -1,0 +1,304 @@
+// Device primitives for the graph and optimizer stages: exclusive scans,
+// a stable LSD radix sort of (u32 key, two u32 payloads), and reductions.
+//
+// Tiling is for CDNA wave64: a 256-thread block (4 waves) owns a tile of 4096
+// elements; each wave owns 1024 consecutive elements, read as 16 coalesced
+// 64-element chunks.
+#include "primitives.hpp"
+
+namespace umapgpu {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr uint64_t kTile = (uint64_t)kThreads * kItems;  // 4096
+
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint64_t t = __shfl_up(v, (unsigned)off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one u64 per thread; returns the exclusive
+// prefix and writes the block total to *total.
+__device__ inline uint64_t block_excl_scan_u64(uint64_t v, uint64_t* total) {
+  __shared__ uint64_t wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t incl = wave_incl_scan_u64(v, lane);
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint64_t before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kThreads / 64; ++i) {
+    if (i < w) before += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const T* __restrict__ in, uint64_t n,
+                                                               uint64_t* __restrict__ partial) {
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint64_t s = 0;
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kThreads + threadIdx.x;  // coalesced
+    if (idx < n) s += (uint64_t)in[idx];
+  }
+  uint64_t tot;
+  (void)block_excl_scan_u64(s, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// out[i] = offset[block] + sum(in[tile_start .. i)); the block holding the
+// last element also writes out[n] = grand total.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void scan_apply_kernel(const T* __restrict__ in, uint64_t n,
+                                                              const uint64_t* __restrict__ offsets,
+                                                              uint64_t* __restrict__ out) {
+  __shared__ uint64_t tile[kTile];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  for (int i = 0; i < kItems; ++i) {  // coalesced stage into LDS
+    uint64_t idx = base + (uint64_t)i * kThreads + threadIdx.x;
+    tile[i * kThreads + threadIdx.x] = (idx < n) ? (uint64_t)in[idx] : 0ull;
+  }
+  __syncthreads();
+  uint64_t local[kItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    local[i] = tile[threadIdx.x * kItems + i];
+    s += local[i];
+  }
+  uint64_t tot;
+  uint64_t pre = block_excl_scan_u64(s, &tot);
+  const uint64_t off = offsets ? offsets[blockIdx.x] : 0ull;
+  uint64_t run = off + pre;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    tile[threadIdx.x * kItems + i] = run;
+    run += local[i];
+  }
+  __syncthreads();
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kThreads + threadIdx.x;
+    if (idx < n) out[idx] = tile[i * kThreads + threadIdx.x];
+  }
+  if (base + kTile >= n && threadIdx.x == 0) out[n] = off + tot;
+}
+
+template <typename T>
+void exclusive_scan_impl(const T* in, uint64_t* out, uint64_t n, hipStream_t s) {
+  if (n == 0) {
+    UMAP_HIP(hipMemsetAsync(out, 0, sizeof(uint64_t), s));
+    return;
+  }
+  const uint64_t nb = ceil_div(n, kTile);
+  if (nb == 1) {
+    scan_apply_kernel<T><<<1, kThreads, 0, s>>>(in, n, nullptr, out);
+    UMAP_LAUNCH_CHECK();
+    return;
+  }
+  DevBuf partial(nb * sizeof(uint64_t)), offs((nb + 1) * sizeof(uint64_t));
+  scan_reduce_kernel<T><<<(unsigned)nb, kThreads, 0, s>>>(in, n, partial.as<uint64_t>());
+  UMAP_LAUNCH_CHECK();
+  exclusive_scan_impl<uint64_t>(partial.as<uint64_t>(), offs.as<uint64_t>(), nb, s);
+  scan_apply_kernel<T><<<(unsigned)nb, kThreads, 0, s>>>(in, n, offs.as<uint64_t>(), out);
+  UMAP_LAUNCH_CHECK();
+  // temporaries are freed by hipFree, which waits for the stream's work
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort.
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint64_t n, int shift,
+                                                           uint32_t* __restrict__ hist, uint32_t nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kThreads + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one 8-bit digit.  Each wave ranks its 1024 elements chunk
+// by chunk: lanes holding the same digit are found with 8 ballots, ranked by
+// popcount of the lower lanes, and the wave's running per-digit counter in
+// LDS is advanced by the group's lowest lane.  A block-level pass then turns
+// per-wave counts plus the global digit offsets into destinations.
+__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ ain, const uint32_t* __restrict__ bin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ aout, uint32_t* __restrict__ bout, uint64_t n, int shift,
+    const uint64_t* __restrict__ offs, uint32_t nb) {
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint64_t wbase[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int i = tid; i < 4 * 256; i += kThreads) (&wcnt[0][0])[i] = 0u;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * 1024;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t key[kItems], va[kItems], vb[kItems], rank[kItems];
+#pragma unroll
+  for (int c = 0; c < kItems; ++c) {
+    const uint64_t idx = base + (uint64_t)c * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t k = valid ? kin[idx] : 0u;
+    key[c] = k;
+    va[c] = valid ? ain[idx] : 0u;
+    vb[c] = valid ? bin[idx] : 0u;
+    const uint32_t dig = (k >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool set = (dig >> bit) & 1u;
+      const uint64_t bb = __ballot(valid && set);
+      m &= set ? bb : ~bb;
+    }
+    const uint32_t r = (uint32_t)__popcll(m & lt);
+    const uint32_t cnt = valid ? wcnt[w][dig] : 0u;
+    rank[c] = cnt + r;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid && r == 0) wcnt[w][dig] = cnt + (uint32_t)__popcll(m);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  {
+    const int d = tid;
+    uint64_t run = offs[(uint64_t)d * nb + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < 4; ++ww) {
+      wbase[ww][d] = run;
+      run += wcnt[ww][d];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kItems; ++c) {
+    const uint64_t idx = base + (uint64_t)c * 64 + lane;
+    if (idx < n) {
+      const uint32_t dig = (key[c] >> shift) & 255u;
+      const uint64_t dst = wbase[w][dig] + rank[c];
+      kout[dst] = key[c];
+      aout[dst] = va[c];
+      bout[dst] = vb[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (max ignoring NaN, like Rust's f32::max).
+__global__ __launch_bounds__(kThreads) void max_partial_kernel(const float* __restrict__ x, uint64_t n,
+                                                               uint64_t stride, float init,
+                                                               float* __restrict__ partial) {
+  float m = init;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads)
+    m = fmaxf(m, x[i * stride]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  __shared__ float ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmaxf(fmaxf(ws[0], ws[1]), fmaxf(ws[2], ws[3]));
+}
+
+__global__ __launch_bounds__(kThreads) void min_partial_kernel(const float* __restrict__ x, uint64_t n,
+                                                               uint64_t stride, float init,
+                                                               float* __restrict__ partial) {
+  float m = init;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads)
+    m = fminf(m, x[i * stride]);
+  for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
+  __shared__ float ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fminf(fminf(ws[0], ws[1]), fminf(ws[2], ws[3]));
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+void exclusive_scan(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s) { exclusive_scan_impl(in, out, n, s); }
+void exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s) { exclusive_scan_impl(in, out, n, s); }
+
+void radix_sort_pairs(uint32_t* keys, uint32_t* va, uint32_t* vb, uint64_t n, uint32_t max_key, hipStream_t s) {
+  if (n <= 1) return;
+  int bits = 0;
+  while (bits < 32 && (max_key >> bits) != 0u) ++bits;
+  if (bits == 0) return;
+  const uint64_t nb = ceil_div(n, kTile);
+  if (nb > 0xffffffffull / 256) throw Error(Err::Unsupported, "radix sort input too large");
+  DevBuf k2(n * 4), a2(n * 4), b2(n * 4);
+  DevBuf hist(256 * nb * 4), offs((256 * nb + 1) * 8);
+  uint32_t *kin = keys, *ain = va, *bin = vb;
+  uint32_t *kout = k2.as<uint32_t>(), *aout = a2.as<uint32_t>(), *bout = b2.as<uint32_t>();
+  int passes = 0;
+  for (int shift = 0; shift < bits; shift += 8, ++passes) {
+    rs_hist_kernel<<<(unsigned)nb, kThreads, 0, s>>>(kin, n, shift, hist.as<uint32_t>(), (uint32_t)nb);
+    UMAP_LAUNCH_CHECK();
+    exclusive_scan(hist.as<uint32_t>(), offs.as<uint64_t>(), 256 * nb, s);
+    rs_scatter_kernel<<<(unsigned)nb, kThreads, 0, s>>>(kin, ain, bin, kout, aout, bout, n, shift,
+                                                         offs.as<uint64_t>(), (uint32_t)nb);
+    UMAP_LAUNCH_CHECK();
+    std::swap(kin, kout);
+    std::swap(ain, aout);
+    std::swap(bin, bout);
+  }
+  if (passes & 1) {
+    UMAP_HIP(hipMemcpyAsync(keys, kin, n * 4, hipMemcpyDeviceToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(va, ain, n * 4, hipMemcpyDeviceToDevice, s));
+    UMAP_HIP(hipMemcpyAsync(vb, bin, n * 4, hipMemcpyDeviceToDevice, s));
+  }
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+float reduce_max_f32(const float* x, uint64_t n, uint64_t stride, float init, hipStream_t s) {
+  if (n == 0) return init;
+  unsigned nb = (unsigned)std::min<uint64_t>(ceil_div(n, kThreads * 8), 2048);
+  DevBuf part(nb * sizeof(float)), out(sizeof(float));
+  max_partial_kernel<<<nb, kThreads, 0, s>>>(x, n, stride, init, part.as<float>());
+  UMAP_LAUNCH_CHECK();
+  max_partial_kernel<<<1, kThreads, 0, s>>>(part.as<float>(), nb, 1, init, out.as<float>());
+  UMAP_LAUNCH_CHECK();
+  float r;
+  UMAP_HIP(hipMemcpyAsync(&r, out.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  return r;
+}
+
+float reduce_min_f32(const float* x, uint64_t n, uint64_t stride, float init, hipStream_t s) {
+  if (n == 0) return init;
+  unsigned nb = (unsigned)std::min<uint64_t>(ceil_div(n, kThreads * 8), 2048);
+  DevBuf part(nb * sizeof(float)), out(sizeof(float));
+  min_partial_kernel<<<nb, kThreads, 0, s>>>(x, n, stride, init, part.as<float>());
+  UMAP_LAUNCH_CHECK();
+  min_partial_kernel<<<1, kThreads, 0, s>>>(part.as<float>(), nb, 1, init, out.as<float>());
+  UMAP_LAUNCH_CHECK();
+  float r;
+  UMAP_HIP(hipMemcpyAsync(&r, out.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  return r;
+}
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return;
+  unsigned nb = (unsigned)std::min<uint64_t>(ceil_div(n, 256), 65535);
+  fill_u32_kernel<<<nb, 256, 0, s>>>(p, n, v);
+  UMAP_LAUNCH_CHECK();
+}
+
+}  // namespace umapgpu

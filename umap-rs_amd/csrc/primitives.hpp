@@ -1,0 +1,24 @@
+// Device primitives shared by the graph and optimizer stages.
+#pragma once
+#include <algorithm>
+#include <utility>
+
+#include "common.hpp"
+
+namespace umapgpu {
+
+// out[0..n] = exclusive prefix sums of in[0..n); out[n] = total.
+void exclusive_scan(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+void exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+
+// Stable ascending sort of keys (all <= max_key) carrying two 32-bit payloads.
+void radix_sort_pairs(uint32_t* keys, uint32_t* va, uint32_t* vb, uint64_t n, uint32_t max_key, hipStream_t s);
+
+// max/min over x[0], x[stride], ... (n values) folded with fmaxf/fminf from
+// `init`; synchronous (returns the host value).
+float reduce_max_f32(const float* x, uint64_t n, uint64_t stride, float init, hipStream_t s);
+float reduce_min_f32(const float* x, uint64_t n, uint64_t stride, float init, hipStream_t s);
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+}  // namespace umapgpu

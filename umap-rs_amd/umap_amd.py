@@ -1,0 +1,736 @@
+"""Python facade over libumapgpu's C ABI, mirroring the umap-rs public API.
+
+Names, argument meanings and error behaviour follow the reference crate
+(umap-rs 0.4.5, paths relative to its tree):
+
+    UmapConfig / ManifoldParams / GraphParams / OptimizationParams  config.rs
+    Metric / MetricType / EuclideanMetric                           metric.rs, distances.rs
+    Umap.new / with_metrics / learn_manifold / fit                  embedding.rs:73-337
+    LearnedManifold.n_vertices / curve_params / graph               manifold.rs:41-54
+    Optimizer.new / step_epochs / *_epochs / embedding / manifold /
+              into_fitted                                           optimizer.rs:74-403
+    FittedUmap.embedding / into_embedding / manifold / config       embedding.rs:462-493
+
+The reference panics on invalid input; here the same message is raised as
+``UmapPanic``.  Every compute call runs on the GPU through libumapgpu.so; if the
+library is missing or no GPU is present the calls raise instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import enum
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libumapgpu.so")
+
+u64, u32, i32, f32, f64 = C.c_uint64, C.c_uint32, C.c_int32, C.c_float, C.c_double
+P = C.c_void_p
+
+OK, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, OVERSTEP, NO_DEVICE, UNSUPPORTED = range(7)
+
+SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY = 0, 1, 2
+
+
+class UmapPanic(RuntimeError):
+    """A reference panic (validation, overstep, shape mismatch)."""
+
+
+class UmapGpuError(RuntimeError):
+    """A device-side failure (HIP error, out of memory, no device)."""
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("n_components", u64),
+        ("min_dist", f32), ("spread", f32), ("has_a", i32), ("a", f32), ("has_b", i32), ("b", f32),
+        ("n_neighbors", u64), ("local_connectivity", f32), ("set_op_mix_ratio", f32),
+        ("has_disconnection_distance", i32), ("disconnection_distance", f32), ("symmetrize", i32),
+        ("has_n_epochs", i32), ("n_epochs", u64), ("learning_rate", f32), ("negative_sample_rate", u64),
+        ("repulsion_strength", f32),
+    ]
+
+
+DISTANCE_FN = C.CFUNCTYPE(f32, C.POINTER(f32), C.POINTER(f32), u64, C.POINTER(f32), P)
+
+_lib = None
+
+
+def lib():
+    """Load libumapgpu.so (built by umap-rs_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python umap-rs_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "umapgpu_version": ([], C.c_char_p),
+            "umapgpu_last_error": ([], C.c_char_p),
+            "umapgpu_set_device": ([i32], i32),
+            "umapgpu_device_count": ([C.POINTER(i32)], i32),
+            "umapgpu_config_default": ([C.POINTER(_Config)], None),
+            "umapgpu_find_ab_params": ([f32, f32, C.POINTER(f32), C.POINTER(f32)], i32),
+            "umapgpu_learn_manifold": ([C.POINTER(_Config), u64, P, u64, u64, u64, P, u64, u64, u64, f32,
+                                        C.POINTER(P)], i32),
+            "umapgpu_learn_manifold_device": ([C.POINTER(_Config), u64, P, P, u64, f32, C.POINTER(P)], i32),
+            "umapgpu_manifold_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(f32), C.POINTER(f32)], i32),
+            "umapgpu_manifold_copy_graph": ([P, P, P, P], i32),
+            "umapgpu_manifold_copy_sigmas_rhos": ([P, P, P], i32),
+            "umapgpu_manifold_disconnections": ([P, C.POINTER(u64), C.POINTER(u64)], i32),
+            "umapgpu_manifold_timings_json": ([P, C.c_char_p, u64], u64),
+            "umapgpu_manifold_from_host": ([u64, u64, P, P, P, P, P, f32, f32, C.POINTER(P)], i32),
+            "umapgpu_manifold_clone": ([P, C.POINTER(P)], i32),
+            "umapgpu_manifold_destroy": ([P], None),
+            "umapgpu_optimizer_new": ([P, P, u64, u64, u64, C.POINTER(_Config), i32, u64, C.POINTER(P)], i32),
+            "umapgpu_optimizer_new_device": ([P, P, u64, u64, u64, C.POINTER(_Config), i32, u64, C.POINTER(P)], i32),
+            "umapgpu_optimizer_step_epochs": ([P, u64], i32),
+            "umapgpu_optimizer_step_epochs_generic": ([P, u64, DISTANCE_FN, P], i32),
+            "umapgpu_optimizer_set_mode": ([P, i32], i32),
+            "umapgpu_optimizer_current_epoch": ([P], u64),
+            "umapgpu_optimizer_total_epochs": ([P], u64),
+            "umapgpu_optimizer_remaining_epochs": ([P], u64),
+            "umapgpu_optimizer_copy_embedding": ([P, P], i32),
+            "umapgpu_optimizer_embedding_device": ([P], P),
+            "umapgpu_optimizer_manifold": ([P], P),
+            "umapgpu_optimizer_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], i32),
+            "umapgpu_optimizer_stats": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(f64)], i32),
+            "umapgpu_optimizer_export_state": ([P, P, P, P, P, P, P, P], i32),
+            "umapgpu_optimizer_export_params": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(f32), C.POINTER(f32),
+                                                 C.POINTER(f64), C.POINTER(i32), C.POINTER(u64)], i32),
+            "umapgpu_optimizer_import_state": ([P, u64, P, P, P, P, P, P, P, u64, u64, u64, f32, f32, f64, i32, u64,
+                                                C.POINTER(P)], i32),
+            "umapgpu_optimizer_into_fitted": ([P, C.POINTER(_Config), C.POINTER(P)], i32),
+            "umapgpu_optimizer_destroy": ([P], None),
+            "umapgpu_fit": ([C.POINTER(_Config), u64, P, u64, u64, u64, P, u64, u64, u64, P, u64, u64, f32, i32,
+                             DISTANCE_FN, P, u64, C.POINTER(P)], i32),
+            "umapgpu_fitted_info": ([P, C.POINTER(u64), C.POINTER(u64)], i32),
+            "umapgpu_fitted_copy_embedding": ([P, P], i32),
+            "umapgpu_fitted_manifold": ([P], P),
+            "umapgpu_fitted_config": ([P, C.POINTER(_Config)], i32),
+            "umapgpu_fitted_from_host": ([P, P, u64, u64, C.POINTER(_Config), C.POINTER(P)], i32),
+            "umapgpu_fitted_destroy": ([P], None),
+            "umapgpu_device_alloc": ([u64, C.POINTER(P)], i32),
+            "umapgpu_device_free": ([P], i32),
+            "umapgpu_memcpy_htod": ([P, P, u64], i32),
+            "umapgpu_memcpy_dtoh": ([P, P, u64], i32),
+            "umapgpu_synchronize": ([], i32),
+            "umapgpu_knn_bruteforce_device": ([P, u64, u64, u64, P, P], i32),
+            "umapgpu_debug_expf_device": ([P, P, u64], i32),
+            "umapgpu_debug_copy_device": ([P, P, u64], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status == OK:
+        return
+    msg = lib().umapgpu_last_error().decode()
+    if status in (INVALID_ARGUMENT, OVERSTEP):
+        raise UmapPanic(msg)
+    raise UmapGpuError(f"[status {status}] {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(P)
+
+
+def device_count() -> int:
+    c = i32(0)
+    _check(lib().umapgpu_device_count(C.byref(c)))
+    return c.value
+
+
+# ---------------------------------------------------------------------------
+# config.rs
+
+@dataclasses.dataclass
+class ManifoldParams:
+    """config.rs:9-52"""
+    min_dist: float = 0.1
+    spread: float = 1.0
+    a: Optional[float] = None
+    b: Optional[float] = None
+
+
+@dataclasses.dataclass
+class GraphParams:
+    """config.rs:59-127"""
+    n_neighbors: int = 15
+    local_connectivity: float = 1.0
+    set_op_mix_ratio: float = 1.0
+    disconnection_distance: Optional[float] = None
+    symmetrize: bool = True
+
+
+@dataclasses.dataclass
+class OptimizationParams:
+    """config.rs:134-180"""
+    n_epochs: Optional[int] = None
+    learning_rate: float = 1.0
+    negative_sample_rate: int = 5
+    repulsion_strength: float = 1.0
+
+
+@dataclasses.dataclass
+class UmapConfig:
+    """config.rs:206-235"""
+    n_components: int = 2
+    manifold: ManifoldParams = dataclasses.field(default_factory=ManifoldParams)
+    graph: GraphParams = dataclasses.field(default_factory=GraphParams)
+    optimization: OptimizationParams = dataclasses.field(default_factory=OptimizationParams)
+
+    def _to_c(self) -> _Config:
+        c = _Config()
+        lib().umapgpu_config_default(C.byref(c))
+        c.n_components = self.n_components
+        m, g, o = self.manifold, self.graph, self.optimization
+        c.min_dist, c.spread = m.min_dist, m.spread
+        c.has_a, c.a = (m.a is not None), (m.a if m.a is not None else 0.0)
+        c.has_b, c.b = (m.b is not None), (m.b if m.b is not None else 0.0)
+        c.n_neighbors = g.n_neighbors
+        c.local_connectivity, c.set_op_mix_ratio = g.local_connectivity, g.set_op_mix_ratio
+        c.has_disconnection_distance = g.disconnection_distance is not None
+        c.disconnection_distance = g.disconnection_distance if g.disconnection_distance is not None else 0.0
+        c.symmetrize = bool(g.symmetrize)
+        c.has_n_epochs = o.n_epochs is not None
+        c.n_epochs = o.n_epochs if o.n_epochs is not None else 0
+        c.learning_rate, c.negative_sample_rate = o.learning_rate, o.negative_sample_rate
+        c.repulsion_strength = o.repulsion_strength
+        return c
+
+    @staticmethod
+    def _from_c(c: _Config) -> "UmapConfig":
+        return UmapConfig(
+            n_components=c.n_components,
+            manifold=ManifoldParams(c.min_dist, c.spread, c.a if c.has_a else None, c.b if c.has_b else None),
+            graph=GraphParams(c.n_neighbors, c.local_connectivity, c.set_op_mix_ratio,
+                              c.disconnection_distance if c.has_disconnection_distance else None, bool(c.symmetrize)),
+            optimization=OptimizationParams(c.n_epochs if c.has_n_epochs else None, c.learning_rate,
+                                            c.negative_sample_rate, c.repulsion_strength))
+
+
+# ---------------------------------------------------------------------------
+# metric.rs / distances.rs
+
+class MetricType(enum.IntEnum):
+    """metric.rs:7-12"""
+    Euclidean = 0
+    Generic = 1
+
+
+class Metric:
+    """trait Metric (metric.rs:19-62)."""
+
+    def metric_type(self) -> MetricType:
+        return MetricType.Generic
+
+    def distance(self, a: np.ndarray, b: np.ndarray):
+        raise NotImplementedError
+
+    def disconnection_threshold(self) -> float:
+        return math.inf
+
+    def squared_distance(self, a, b):
+        return None
+
+
+class EuclideanMetric(Metric):
+    """distances.rs:10-50"""
+
+    def metric_type(self):
+        return MetricType.Euclidean
+
+    def distance(self, x, y):
+        x = np.asarray(x, np.float32)
+        y = np.asarray(y, np.float32)
+        s = np.float32(0.0)
+        for i in range(x.size):
+            d = np.float32(x[i] - y[i])
+            s = np.float32(s + d * d)
+        dist = np.float32(np.sqrt(s))
+        return dist, ((x - y) / np.float32(dist + np.float32(1e-6))).astype(np.float32)
+
+    def squared_distance(self, x, y):
+        d = np.asarray(x, np.float32) - np.asarray(y, np.float32)
+        return np.float32(np.sum(d * d))
+
+
+def _distance_callback(metric: Metric):
+    def cb(a, b, dim, grad, user):
+        av = np.ctypeslib.as_array(a, shape=(dim,)).copy()
+        bv = np.ctypeslib.as_array(b, shape=(dim,)).copy()
+        d, g = metric.distance(av, bv)
+        gv = np.ctypeslib.as_array(grad, shape=(dim,))
+        gv[:] = np.asarray(g, np.float32)
+        return float(d)
+    return DISTANCE_FN(cb)
+
+
+# ---------------------------------------------------------------------------
+
+class SparseMat:
+    """CsMatI<f32, u32, usize> (umap/fuzzy_simplicial_set.rs:18) on the host."""
+
+    def __init__(self, n: int, indptr: np.ndarray, indices: np.ndarray, data: np.ndarray):
+        self._n, self._indptr, self._indices, self._data = n, indptr, indices, data
+
+    def nnz(self) -> int:
+        return int(self._indptr[-1])
+
+    def shape(self):
+        return (self._n, self._n)
+
+    def indptr(self):
+        return self._indptr
+
+    def indices(self):
+        return self._indices
+
+    def data(self):
+        return self._data
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self._n, self._n), np.float32)
+        for r in range(self._n):
+            s, e = int(self._indptr[r]), int(self._indptr[r + 1])
+            d[r, self._indices[s:e]] = self._data[s:e]
+        return d
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self, h):
+        self._h = h
+        self._owned = True
+
+    def _handle(self):
+        if self._h is None:
+            raise UmapPanic(f"{type(self).__name__} was moved (consumed) and can no longer be used")
+        return self._h
+
+    def _release(self):
+        h = self._handle()
+        self._h = None
+        self._owned = False
+        return h
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._owned and _lib is not None:
+                getattr(_lib, self._destroy)(self._h)
+        except Exception:
+            pass
+
+
+class LearnedManifold(_Handle):
+    """LearnedManifold (manifold.rs:15-54), device resident."""
+    _destroy = "umapgpu_manifold_destroy"
+
+    @staticmethod
+    def _borrowed(h, owner):
+        m = LearnedManifold(h)
+        m._owned = False
+        m._owner = owner
+        return m
+
+    def _info(self):
+        nv, nnz, a, b = u64(), u64(), f32(), f32()
+        _check(lib().umapgpu_manifold_info(self._handle(), C.byref(nv), C.byref(nnz), C.byref(a), C.byref(b)))
+        return nv.value, nnz.value, np.float32(a.value), np.float32(b.value)
+
+    def n_vertices(self) -> int:
+        return self._info()[0]
+
+    def curve_params(self):
+        _, _, a, b = self._info()
+        return a, b
+
+    def graph(self) -> SparseMat:
+        nv, nnz, _, _ = self._info()
+        ip = np.zeros(nv + 1, np.uint64)
+        ix = np.zeros(max(nnz, 1), np.uint32)
+        dv = np.zeros(max(nnz, 1), np.float32)
+        _check(lib().umapgpu_manifold_copy_graph(self._handle(), _ptr(ip), _ptr(ix), _ptr(dv)))
+        return SparseMat(nv, ip, ix[:nnz], dv[:nnz])
+
+    def sigmas_rhos(self):
+        nv = self.n_vertices()
+        s = np.zeros(max(nv, 1), np.float32)
+        r = np.zeros(max(nv, 1), np.float32)
+        _check(lib().umapgpu_manifold_copy_sigmas_rhos(self._handle(), _ptr(s), _ptr(r)))
+        return s[:nv], r[:nv]
+
+    def disconnections(self):
+        er, vd = u64(), u64()
+        _check(lib().umapgpu_manifold_disconnections(self._handle(), C.byref(er), C.byref(vd)))
+        return er.value, vd.value
+
+    def timings(self) -> dict:
+        import json
+        n = lib().umapgpu_manifold_timings_json(self._handle(), None, 0)
+        buf = C.create_string_buffer(int(n))
+        lib().umapgpu_manifold_timings_json(self._handle(), buf, n)
+        return json.loads(buf.value.decode())
+
+    def clone(self) -> "LearnedManifold":
+        out = P()
+        _check(lib().umapgpu_manifold_clone(self._handle(), C.byref(out)))
+        return LearnedManifold(out)
+
+    # serde: host arrays (manifold.rs:14)
+    def to_state(self) -> dict:
+        g = self.graph()
+        s, r = self.sigmas_rhos()
+        nv, _, a, b = self._info()
+        return {"indptr": g.indptr(), "indices": g.indices(), "data": g.data(), "sigmas": s, "rhos": r,
+                "n_vertices": np.uint64(nv), "a": np.float32(a), "b": np.float32(b)}
+
+    @staticmethod
+    def from_state(st: dict) -> "LearnedManifold":
+        ip = np.ascontiguousarray(st["indptr"], np.uint64)
+        ix = np.ascontiguousarray(st["indices"], np.uint32)
+        dv = np.ascontiguousarray(st["data"], np.float32)
+        s = np.ascontiguousarray(st["sigmas"], np.float32)
+        r = np.ascontiguousarray(st["rhos"], np.float32)
+        n = int(st["n_vertices"])
+        out = P()
+        _check(lib().umapgpu_manifold_from_host(n, int(ip[-1]), _ptr(ip), _ptr(ix), _ptr(dv), _ptr(s), _ptr(r),
+                                                float(st["a"]), float(st["b"]), C.byref(out)))
+        return LearnedManifold(out)
+
+
+class FittedUmap(_Handle):
+    """FittedUmap (embedding.rs:439-523)."""
+    _destroy = "umapgpu_fitted_destroy"
+
+    def _shape(self):
+        n, d = u64(), u64()
+        _check(lib().umapgpu_fitted_info(self._handle(), C.byref(n), C.byref(d)))
+        return n.value, d.value
+
+    def embedding(self) -> np.ndarray:
+        n, d = self._shape()
+        out = np.zeros((n, d), np.float32)
+        _check(lib().umapgpu_fitted_copy_embedding(self._handle(), _ptr(out)))
+        return out
+
+    def into_embedding(self) -> np.ndarray:
+        e = self.embedding()
+        lib().umapgpu_fitted_destroy(self._release())
+        return e
+
+    def manifold(self) -> LearnedManifold:
+        return LearnedManifold._borrowed(lib().umapgpu_fitted_manifold(self._handle()), self)
+
+    def config(self) -> UmapConfig:
+        c = _Config()
+        _check(lib().umapgpu_fitted_config(self._handle(), C.byref(c)))
+        return UmapConfig._from_c(c)
+
+    def transform(self, new_data, new_knn_indices, new_knn_dists):
+        raise NotImplementedError("Transform not yet implemented - contributions welcome!")  # :516-523
+
+    def to_state(self) -> dict:
+        st = {"embedding": self.embedding(), "config": dataclasses.asdict(self.config())}
+        st["manifold"] = self.manifold().to_state()
+        return st
+
+    @staticmethod
+    def from_state(st: dict) -> "FittedUmap":
+        m = LearnedManifold.from_state(st["manifold"])
+        cfg = st["config"]
+        cfg = UmapConfig(cfg["n_components"], ManifoldParams(**cfg["manifold"]), GraphParams(**cfg["graph"]),
+                         OptimizationParams(**cfg["optimization"])) if isinstance(cfg, dict) else cfg
+        e = np.ascontiguousarray(st["embedding"], np.float32)
+        out = P()
+        c = cfg._to_c()
+        _check(lib().umapgpu_fitted_from_host(m._handle(), _ptr(e), e.shape[0], e.shape[1], C.byref(c), C.byref(out)))
+        m._release()
+        return FittedUmap(out)
+
+
+class Optimizer(_Handle):
+    """Optimizer (optimizer.rs:28-403), device resident."""
+    _destroy = "umapgpu_optimizer_destroy"
+
+    @staticmethod
+    def new(manifold: LearnedManifold, init, total_epochs: int, config: UmapConfig,
+            metric_type: MetricType = MetricType.Euclidean, seed: int = 0) -> "Optimizer":
+        """Optimizer::new (optimizer.rs:74-290); consumes `manifold`."""
+        init = np.ascontiguousarray(init, dtype=np.float32)
+        if init.ndim != 2:
+            raise UmapPanic("init must be a 2-D array")
+        out = P()
+        c = config._to_c()
+        _check(lib().umapgpu_optimizer_new(manifold._handle(), _ptr(init), init.shape[0], init.shape[1],
+                                           total_epochs, C.byref(c), int(metric_type), seed, C.byref(out)))
+        manifold._release()
+        return Optimizer(out)
+
+    @staticmethod
+    def new_device(manifold: LearnedManifold, init_device: int, n: int, dim: int, total_epochs: int,
+                   config: UmapConfig, metric_type: MetricType = MetricType.Euclidean, seed: int = 0):
+        out = P()
+        c = config._to_c()
+        _check(lib().umapgpu_optimizer_new_device(manifold._handle(), P(init_device), n, dim, total_epochs,
+                                                  C.byref(c), int(metric_type), seed, C.byref(out)))
+        manifold._release()
+        return Optimizer(out)
+
+    def step_epochs(self, n: int, output_metric: Optional[Metric] = None) -> None:
+        """Optimizer::step_epochs (optimizer.rs:297-366)."""
+        h = self._handle()
+        mt = self.export_params()["metric_type"]
+        if mt == MetricType.Generic:
+            if output_metric is None:
+                raise UmapPanic("MetricType::Generic needs the output metric")
+            cb = _distance_callback(output_metric)
+            _check(lib().umapgpu_optimizer_step_epochs_generic(h, n, cb, None))
+        else:
+            _check(lib().umapgpu_optimizer_step_epochs(h, n))
+
+    def set_mode(self, mode: int) -> None:
+        _check(lib().umapgpu_optimizer_set_mode(self._handle(), mode))
+
+    def current_epoch(self) -> int:
+        return lib().umapgpu_optimizer_current_epoch(self._handle())
+
+    def total_epochs(self) -> int:
+        return lib().umapgpu_optimizer_total_epochs(self._handle())
+
+    def remaining_epochs(self) -> int:
+        return lib().umapgpu_optimizer_remaining_epochs(self._handle())
+
+    def info(self):
+        ne, nv, d = u64(), u64(), u64()
+        _check(lib().umapgpu_optimizer_info(self._handle(), C.byref(ne), C.byref(nv), C.byref(d)))
+        return ne.value, nv.value, d.value
+
+    def embedding(self) -> np.ndarray:
+        _, nv, d = self.info()
+        out = np.zeros((nv, d), np.float32)
+        _check(lib().umapgpu_optimizer_copy_embedding(self._handle(), _ptr(out)))
+        return out
+
+    def embedding_device_ptr(self) -> int:
+        return lib().umapgpu_optimizer_embedding_device(self._handle())
+
+    def manifold(self) -> LearnedManifold:
+        return LearnedManifold._borrowed(lib().umapgpu_optimizer_manifold(self._handle()), self)
+
+    def stats(self):
+        p, n, ms = u64(), u64(), f64()
+        _check(lib().umapgpu_optimizer_stats(self._handle(), C.byref(p), C.byref(n), C.byref(ms)))
+        return p.value, n.value, ms.value
+
+    def export_params(self) -> dict:
+        cur, tot, g, ia, nsr, mt, seed = u64(), u64(), f32(), f32(), f64(), i32(), u64()
+        _check(lib().umapgpu_optimizer_export_params(self._handle(), C.byref(cur), C.byref(tot), C.byref(g),
+                                                     C.byref(ia), C.byref(nsr), C.byref(mt), C.byref(seed)))
+        return {"current_epoch": cur.value, "total_epochs": tot.value, "gamma": np.float32(g.value),
+                "initial_alpha": np.float32(ia.value), "negative_sample_rate": nsr.value,
+                "metric_type": MetricType(mt.value), "seed": seed.value}
+
+    def export_state(self) -> dict:
+        ne, nv, d = self.info()
+        n1 = max(ne, 1)
+        arr = {k: np.zeros(n1, t) for k, t in (("head", np.uint32), ("tail", np.uint32),
+                                               ("epochs_per_sample", np.float64),
+                                               ("epoch_of_next_sample", np.float64),
+                                               ("epochs_per_negative_sample", np.float64),
+                                               ("epoch_of_next_negative_sample", np.float64))}
+        emb = np.zeros((nv, d), np.float32)
+        _check(lib().umapgpu_optimizer_export_state(
+            self._handle(), _ptr(arr["head"]), _ptr(arr["tail"]), _ptr(arr["epochs_per_sample"]),
+            _ptr(arr["epoch_of_next_sample"]), _ptr(arr["epochs_per_negative_sample"]),
+            _ptr(arr["epoch_of_next_negative_sample"]), _ptr(emb)))
+        out = {k: v[:ne] for k, v in arr.items()}
+        out["embedding"] = emb
+        return out
+
+    # serde (optimizer.rs:27): the full checkpoint as host arrays
+    def to_state(self) -> dict:
+        st = self.export_state()
+        st.update(self.export_params())
+        st["manifold"] = self.manifold().to_state()
+        return st
+
+    @staticmethod
+    def from_state(st: dict) -> "Optimizer":
+        m = LearnedManifold.from_state(st["manifold"])
+        a = {k: np.ascontiguousarray(st[k]) for k in ("head", "tail", "epochs_per_sample", "epoch_of_next_sample",
+                                                       "epochs_per_negative_sample",
+                                                       "epoch_of_next_negative_sample")}
+        emb = np.ascontiguousarray(st["embedding"], np.float32)
+        out = P()
+        _check(lib().umapgpu_optimizer_import_state(
+            m._handle(), a["head"].size, _ptr(a["head"].astype(np.uint32)), _ptr(a["tail"].astype(np.uint32)),
+            _ptr(a["epochs_per_sample"]), _ptr(a["epoch_of_next_sample"]), _ptr(a["epochs_per_negative_sample"]),
+            _ptr(a["epoch_of_next_negative_sample"]), _ptr(emb), emb.shape[1], int(st["current_epoch"]),
+            int(st["total_epochs"]), float(st["gamma"]), float(st["initial_alpha"]),
+            float(st["negative_sample_rate"]), int(st["metric_type"]), int(st["seed"]), C.byref(out)))
+        m._release()
+        return Optimizer(out)
+
+    def into_fitted(self, config: UmapConfig) -> FittedUmap:
+        """Optimizer::into_fitted (optimizer.rs:397-403); consumes self."""
+        out = P()
+        c = config._to_c()
+        _check(lib().umapgpu_optimizer_into_fitted(self._handle(), C.byref(c), C.byref(out)))
+        self._release()
+        return FittedUmap(out)
+
+
+def _knn_views(knn_indices, knn_dists):
+    idx = np.asarray(knn_indices)
+    dst = np.asarray(knn_dists)
+    if idx.dtype != np.uint32:
+        idx = idx.astype(np.uint32)
+    if dst.dtype != np.float32:
+        dst = dst.astype(np.float32)
+    if idx.ndim != 2 or dst.ndim != 2:
+        raise UmapPanic("knn arrays must be 2-D")
+
+    def rowview(a):
+        if a.strides[1] != a.itemsize or a.strides[0] % a.itemsize:
+            a = np.ascontiguousarray(a)
+        return a, a.strides[0] // a.itemsize if a.shape[0] > 1 else a.shape[1]
+    idx, si = rowview(idx)
+    dst, sd = rowview(dst)
+    return idx, si, dst, sd
+
+
+class Umap:
+    """Umap (embedding.rs:58-430)."""
+
+    def __init__(self, config: UmapConfig, metric: Optional[Metric] = None, output_metric: Optional[Metric] = None):
+        self.config = config
+        self.metric = metric or EuclideanMetric()
+        self.output_metric = output_metric or EuclideanMetric()
+
+    @staticmethod
+    def new(config: UmapConfig) -> "Umap":
+        return Umap(config)
+
+    @staticmethod
+    def with_metrics(config: UmapConfig, metric: Metric, output_metric: Metric) -> "Umap":
+        return Umap(config, metric, output_metric)
+
+    def learn_manifold(self, data, knn_indices, knn_dists) -> LearnedManifold:
+        """Umap::learn_manifold (embedding.rs:149-240); only data.shape[0] is used."""
+        n_samples = int(np.shape(data)[0])
+        idx, si, dst, sd = _knn_views(knn_indices, knn_dists)
+        out = P()
+        c = self.config._to_c()
+        _check(lib().umapgpu_learn_manifold(C.byref(c), n_samples, _ptr(idx), idx.shape[0], idx.shape[1], si,
+                                            _ptr(dst), dst.shape[0], dst.shape[1], sd,
+                                            float(self.metric.disconnection_threshold()), C.byref(out)))
+        return LearnedManifold(out)
+
+    def fit(self, data, knn_indices, knn_dists, init, seed: int = 0) -> FittedUmap:
+        """Umap::fit (embedding.rs:274-337)."""
+        n_samples = int(np.shape(data)[0])
+        idx, si, dst, sd = _knn_views(knn_indices, knn_dists)
+        init = np.ascontiguousarray(init, np.float32)
+        out = P()
+        c = self.config._to_c()
+        mt = self.output_metric.metric_type()
+        cb = _distance_callback(self.output_metric) if mt == MetricType.Generic else DISTANCE_FN()
+        _check(lib().umapgpu_fit(C.byref(c), n_samples, _ptr(idx), idx.shape[0], idx.shape[1], si, _ptr(dst),
+                                 dst.shape[0], dst.shape[1], sd, _ptr(init), init.shape[0], init.shape[1],
+                                 float(self.metric.disconnection_threshold()), int(mt), cb, None, seed,
+                                 C.byref(out)))
+        return FittedUmap(out)
+
+
+def find_ab_params(spread: float, min_dist: float):
+    a, b = f32(), f32()
+    _check(lib().umapgpu_find_ab_params(spread, min_dist, C.byref(a), C.byref(b)))
+    return np.float32(a.value), np.float32(b.value)
+
+
+# ---------------------------------------------------------------------------
+# device buffers and the kNN producer (harness utilities)
+
+class DeviceArray:
+    """A device allocation owned by Python (for inputs kept resident in HBM)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = P()
+        _check(lib().umapgpu_device_alloc(max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_host(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.shape, a.dtype)
+        if a.nbytes:
+            _check(lib().umapgpu_memcpy_htod(P(d.ptr), _ptr(a), a.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().umapgpu_memcpy_dtoh(_ptr(out), P(self.ptr), self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().umapgpu_device_free(P(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.umapgpu_device_free(P(self.ptr))
+        except Exception:
+            pass
+
+
+def knn_bruteforce_device(data: DeviceArray, k: int):
+    n, dim = data.shape
+    idx = DeviceArray((n, k), np.uint32)
+    dst = DeviceArray((n, k), np.float32)
+    _check(lib().umapgpu_knn_bruteforce_device(P(data.ptr), n, dim, k, P(idx.ptr), P(dst.ptr)))
+    return idx, dst
+
+
+def learn_manifold_device(config: UmapConfig, knn_idx: DeviceArray, knn_dst: DeviceArray,
+                          metric: Optional[Metric] = None) -> LearnedManifold:
+    n, k = knn_idx.shape
+    out = P()
+    c = config._to_c()
+    thr = (metric or EuclideanMetric()).disconnection_threshold()
+    _check(lib().umapgpu_learn_manifold_device(C.byref(c), n, P(knn_idx.ptr), P(knn_dst.ptr), k, float(thr),
+                                               C.byref(out)))
+    return LearnedManifold(out)
+
+
+def debug_copy(src: DeviceArray, dst: DeviceArray, nbytes: int) -> None:
+    _check(lib().umapgpu_debug_copy_device(P(src.ptr), P(dst.ptr), nbytes))
+
+
+def synchronize() -> None:
+    _check(lib().umapgpu_synchronize())
+
+
+def debug_expf(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    dx = DeviceArray.from_host(x)
+    dy = DeviceArray(x.shape, np.float32)
+    _check(lib().umapgpu_debug_expf_device(P(dx.ptr), P(dy.ptr), x.size))
+    return dy.to_host()

@@ -1,0 +1,106 @@
+"""Embedding-quality evaluators (the reference defines none).
+
+* fuzzy_cross_entropy: the UMAP objective on the *unfiltered* symmetric graph:
+      sum_{(i,j) in graph} w log(w/v) + (1-w) log((1-w)/(1-v))
+    + (n^2 - nnz)/M * sum_{M random non-edge pairs} -log(1-v)
+  with v = 1/(1 + a*|y_i - y_j|^(2b)) clamped to [1e-12, 1-1e-12].
+* knn_preservation: mean over a fixed subsample of |kNN_high(i) & kNN_2d(i)|/k,
+  kNN_high from the given kNN graph (self excluded), kNN_2d exact over all n.
+* trustworthiness: the rank-based formula (Venna & Kaski 2001), as in
+  sklearn.manifold.trustworthiness, evaluated on a fixed subsample of queries
+  with ranks taken over all n points.
+
+All use float64 on the host and fixed seeds, so repeated calls agree.  Rows
+whose embedding is NaN (fit marks disconnected vertices that way) are skipped.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _v(y, i, j, a, b):
+    d2 = ((y[i] - y[j]) ** 2).sum(-1)
+    v = 1.0 / (1.0 + a * np.power(d2, b))  # |d|^(2b) = (d^2)^b
+    return np.clip(v, 1e-12, 1.0 - 1e-12)
+
+
+def fuzzy_cross_entropy(indptr, indices, data, embedding, a, b, n_random=200_000, seed=3):
+    y = np.asarray(embedding, np.float64)
+    n = y.shape[0]
+    ip = np.asarray(indptr, np.int64)
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    cols = np.asarray(indices, np.int64)
+    w = np.clip(np.asarray(data, np.float64), 1e-12, 1.0 - 1e-12)
+    ok = np.isfinite(y[rows]).all(1) & np.isfinite(y[cols]).all(1)
+    rows, cols, w = rows[ok], cols[ok], w[ok]
+    v = _v(y, rows, cols, float(a), float(b))
+    attract = float(np.sum(w * np.log(w / v) + (1.0 - w) * np.log((1.0 - w) / (1.0 - v))))
+    rng = np.random.default_rng(seed)
+    i = rng.integers(0, n, n_random)
+    j = rng.integers(0, n, n_random)
+    keep = (i != j) & np.isfinite(y[i]).all(1) & np.isfinite(y[j]).all(1)
+    i, j = i[keep], j[keep]
+    rep = -np.log(1.0 - _v(y, i, j, float(a), float(b)))
+    scale = (float(n) * n - len(cols)) / max(len(i), 1)
+    return attract + scale * float(rep.sum())
+
+
+def _subsample(n, m, seed):
+    if m >= n:
+        return np.arange(n)
+    return np.sort(np.random.default_rng(seed).choice(n, m, replace=False))
+
+
+def _knn_rows(y, q, k):
+    """Exact kNN of rows q of y among all rows of y (self excluded)."""
+    y = np.asarray(y, np.float64)
+    sq = (y * y).sum(1)
+    out = np.zeros((len(q), k), np.int64)
+    for s in range(0, len(q), 512):
+        qq = q[s:s + 512]
+        d2 = sq[qq, None] + sq[None, :] - 2.0 * y[qq] @ y.T
+        d2[np.arange(len(qq)), qq] = np.inf
+        d2[~np.isfinite(d2)] = np.inf
+        part = np.argpartition(d2, k, axis=1)[:, :k]
+        order = np.argsort(np.take_along_axis(d2, part, 1), axis=1, kind="stable")
+        out[s:s + 512] = np.take_along_axis(part, order, 1)
+    return out
+
+
+def knn_preservation(knn_indices, embedding, k=15, m=10_000, seed=3):
+    idx = np.asarray(knn_indices)
+    n = idx.shape[0]
+    y = np.asarray(embedding, np.float64)
+    q = _subsample(n, m, seed)
+    q = q[np.isfinite(y[q]).all(1)]
+    yy = np.where(np.isfinite(y), y, 1e30)
+    low = _knn_rows(yy, q, k)
+    tot = 0.0
+    for r, i in enumerate(q):
+        high = [c for c in idx[i] if c != i][:k]
+        tot += len(set(high) & set(low[r].tolist())) / float(k)
+    return tot / max(len(q), 1)
+
+
+def trustworthiness(x, embedding, k=15, m=10_000, seed=3):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(embedding, np.float64)
+    n = x.shape[0]
+    q = _subsample(n, m, seed)
+    q = q[np.isfinite(y[q]).all(1)]
+    yy = np.where(np.isfinite(y), y, 1e30)
+    low = _knn_rows(yy, q, k)
+    sqx = (x * x).sum(1)
+    penalty = 0.0
+    for s in range(0, len(q), 256):
+        qq = q[s:s + 256]
+        d2 = sqx[qq, None] + sqx[None, :] - 2.0 * x[qq] @ x.T
+        d2[np.arange(len(qq)), qq] = -np.inf  # self gets rank 0
+        order = np.argsort(d2, axis=1, kind="stable")
+        ranks = np.empty_like(order)
+        ranks[np.arange(len(qq))[:, None], order] = np.arange(n)[None, :]
+        for r in range(len(qq)):
+            rk = ranks[r, low[s + r]]
+            penalty += float(np.sum(np.maximum(rk - k, 0)))
+    mm = len(q)
+    return 1.0 - 2.0 / (mm * k * (2.0 * n - 3.0 * k - 1.0)) * penalty
