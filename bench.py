@@ -68,6 +68,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's SGD sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-quality", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (scales profiler traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +112,15 @@ def main() -> int:
     log(f"learn_manifold: {graph_s:.3f}s wall, nnz={gt['nnz']} (directed {gt['nnz_directed']})")
     d_init = U.DeviceArray.from_host(init)
     U.synchronize()
+
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src, dst = U.DeviceArray((nb,), np.uint8), U.DeviceArray((nb,), np.uint8)
+        U.debug_copy(src, dst, nb)
+        U.synchronize()
+        log(f"calibration copy of {nb} bytes done")
+        src.free()
+        dst.free()
 
     mode = U.SGD_HOGWILD if args.mode == "hogwild" else U.SGD_HOGWILD_RACY
     times, kernel_ms, updates, negs, prep_ms = [], [], [], [], []

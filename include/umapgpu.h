@@ -256,6 +256,8 @@ UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_devic
 UMAPGPU_API umapgpu_status umapgpu_debug_copy_device(const void* src_device, void* dst_device, uint64_t bytes);
 /* Test hook: y[i] = expf(x[i]) through the device's glibc-equivalent expf. */
 UMAPGPU_API umapgpu_status umapgpu_debug_expf_device(const float* x_device, float* y_device, uint64_t n);
+/* Test hook: y[i] = powf(x[i], e) through the device's glibc-equivalent powf. */
+UMAPGPU_API umapgpu_status umapgpu_debug_powf_device(const float* x_device, float e, float* y_device, uint64_t n);
 
 #ifdef __cplusplus
 }

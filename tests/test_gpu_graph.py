@@ -59,7 +59,7 @@ def test_device_expf_matches_glibc(gpu):
                         rng.uniform(0, 90, 500_000)]).astype(F)
     x = np.concatenate([x, np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 88.72284, -103.97208], F)])
     y = gpu.debug_expf(x)
-    ref = np.exp(x.astype(np.float64))  # glibc expf via numpy float32 would also do; compare with libc below
+    ref = np.exp(x.astype(np.float64))
     import ctypes
     libm = ctypes.CDLL("libm.so.6")
     libm.expf.restype = ctypes.c_float
@@ -67,7 +67,24 @@ def test_device_expf_matches_glibc(gpu):
     sample = np.concatenate([np.arange(0, x.size, 97), np.arange(x.size - 7, x.size)])
     g = np.array([libm.expf(float(v)) for v in x[sample]], F)
     assert np.array_equal(y[sample].view(np.uint32)[~np.isnan(g)], g.view(np.uint32)[~np.isnan(g)])
-    assert np.allclose(y[np.isfinite(ref)], ref[np.isfinite(ref)], rtol=1e-6, atol=1e-44)
+    normal = (ref > 1.2e-38) & (ref < 3.4e38)  # f32 normal range; edges checked bitwise above
+    assert np.allclose(y[normal], ref[normal], rtol=1e-6, atol=0)
+
+
+def test_device_powf_matches_glibc(gpu):
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.powf.restype = ctypes.c_float
+    libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(0, 4, 200_000), 10.0 ** rng.uniform(-40, 38, 200_000),
+                        np.array([0.0, np.inf, 1.0, 1e-45, np.nan])]).astype(F)
+    for e in (0.9002582, 0.8794114, 1.7, 0.5):
+        y = gpu.debug_powf(x, e)
+        sample = np.arange(0, x.size, 37)
+        g = np.array([libm.powf(float(v), e) for v in x[sample]], F)
+        ok = ~np.isnan(g)
+        assert np.array_equal(y[sample].view(np.uint32)[ok], g.view(np.uint32)[ok])
 
 
 def test_ring_fixture(gpu, oracle):
@@ -112,8 +129,10 @@ def test_disconnection_distance_and_sentinels(gpu, oracle):
     _, om = _compare(gpu, oracle, idx, d, disconnection_distance=thr)
     assert om.edges_removed > 0
     d2 = d.copy()
-    d2[20, :] = np.inf           # a fully disconnected row (+inf >= +inf)
-    _, om = _compare(gpu, oracle, idx, d2)
+    d2[20, :] = np.inf           # row 20 keeps none of its slots (+inf >= +inf)...
+    idx2 = np.where(idx == 20, np.uint32(0xFFFFFFFF), idx)  # ...and no row points at it
+    idx2[20, 0] = 20
+    _, om = _compare(gpu, oracle, idx2, d2)
     assert om.vertices_disconnected >= 1
 
 

@@ -1,0 +1,71 @@
+"""The device expf/powf restatement (umap-rs_amd/csrc/libm_glibc.hpp) equals
+glibc's expf/powf -- what Rust's f32::exp / f32::powf call in the reference --
+bit for bit.  Compiled as host code with g++ and compared on the CPU:
+  expf: every float in [-104, -0] (all membership and bisection arguments are
+        <= 0), plus a dense stride over every other bit pattern;
+  powf: every positive float for the exponents the SGD uses (b of the default
+        curve fits), a dense stride for other exponents, and special values.
+"""
+import os
+import struct
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("libm") / "libm_glibc_check")
+    subprocess.run(["g++", "-O2", "-fopenmp", "-ffp-contract=off", "-fno-builtin", "-std=c++17",
+                    "-I", os.path.join(REPO, "umap-rs_amd", "csrc"), os.path.join(REPO, "tests", "libm_glibc_check.cpp"),
+                    "-o", exe, "-lm"], check=True)
+    return exe
+
+
+def bits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def run(exe, *args):
+    out = subprocess.run([exe, *map(str, args)], check=True, capture_output=True, text=True).stdout
+    tested, mism = map(int, out.split())
+    return tested, mism
+
+
+@pytest.mark.slow
+def test_expf_exhaustive_nonpositive(checker):
+    tested, mism = run(checker, "exp", 0x80000000, bits(-104.0), 1)
+    assert tested > 1_100_000_000 and mism == 0
+
+
+def test_expf_strided_everything_else(checker):
+    tested, mism = run(checker, "exp", 0, 0xffffffff, 4099)
+    assert tested > 1_000_000 and mism == 0
+
+
+def test_expf_special_values(checker):
+    for x in (0.0, -0.0, float("inf"), float("-inf"), 88.72283, 88.72284, -103.97207, -103.97208, -87.0, 1e-30):
+        b = bits(x)
+        assert run(checker, "exp", b, b, 1)[1] == 0
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("y", ["0.9002582", "0.8794114"])  # b for min_dist 0.1 and 0.05
+def test_powf_exhaustive_positive(checker, y):
+    tested, mism = run(checker, "pow", 1, 0x7f7fffff, 1, y)
+    assert tested > 2_000_000_000 and mism == 0
+
+
+@pytest.mark.parametrize("y", ["0.858017206", "0.98140067", "0.5", "1.7", "3.1", "0.001", "10", "-0.75", "2", "-3"])
+def test_powf_strided(checker, y):
+    tested, mism = run(checker, "pow", 0, 0xffffffff, 997, y)
+    assert tested > 4_000_000 and mism == 0
+
+
+def test_powf_special_values(checker):
+    for y in ("0", "-0", "inf", "-inf", "nan", "1", "-1", "0.5", "3"):
+        for x in (0.0, -0.0, 1.0, -1.0, float("inf"), float("-inf"), 1e-45, -2.0, 0.5, 3.0e38):
+            b = bits(x)
+            assert run(checker, "pow", b, b, 1, y)[1] == 0, (x, y)

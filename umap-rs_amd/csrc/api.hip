@@ -876,6 +876,13 @@ umapgpu_status umapgpu_debug_copy_device(const void* src, void* dst, uint64_t by
   });
 }
 
+umapgpu_status umapgpu_debug_powf_device(const float* x, float e, float* y, uint64_t n) {
+  return guarded([&] {
+    ensure_device();
+    device_powf(x, e, y, n, stream());
+  });
+}
+
 umapgpu_status umapgpu_debug_expf_device(const float* x, float* y, uint64_t n) {
   return guarded([&] {
     ensure_device();

@@ -16,7 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "graph.hpp"
-#include "libm_expf.hpp"
+#include "libm_glibc.hpp"
 #include "primitives.hpp"
 
 namespace umapgpu {
@@ -419,6 +419,11 @@ __global__ void expf_kernel(const float* __restrict__ x, float* __restrict__ y, 
     y[i] = libm_expf(x[i]);
 }
 
+__global__ void powf_kernel(const float* __restrict__ x, float e, float* __restrict__ y, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] = libm_powf(x[i], e);
+}
+
 unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)ceil_div(n, block); }
 
 struct EventTimer {
@@ -609,6 +614,13 @@ void device_copy_calibration(const void* src, void* dst, uint64_t bytes, hipStre
   if (!n16) return;
   copy_calibration_kernel<<<(unsigned)std::min<uint64_t>(ceil_div(n16, 256), 1u << 16), 256, 0, s>>>(
       static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  UMAP_LAUNCH_CHECK();
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void device_powf(const float* x, float e, float* y, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  powf_kernel<<<(unsigned)std::min<uint64_t>(grid_for(n, 256), 8192), 256, 0, s>>>(x, e, y, n);
   UMAP_LAUNCH_CHECK();
   UMAP_HIP(hipStreamSynchronize(s));
 }

@@ -53,5 +53,7 @@ void device_copy_calibration(const void* src, void* dst, uint64_t bytes, hipStre
 
 // Test hook: exp of every x[i] through the device glibc-equivalent expf.
 void device_expf(const float* x, float* y, uint64_t n, hipStream_t s);
+// Test hook: y[i] = powf(x[i], e) through the device glibc-equivalent powf.
+void device_powf(const float* x, float e, float* y, uint64_t n, hipStream_t s);
 
 }  // namespace umapgpu

@@ -1,0 +1,218 @@
+// Single-precision exp and pow that reproduce glibc's expf and powf bit for
+// bit, on the host and on the device.
+//
+// The reference computes every membership weight and sigma-bisection term
+// with Rust's f32::exp and every SGD force with f32::powf; on x86-64 Linux
+// those are glibc's expf and powf, and glibc picks their FMA builds on every
+// CPU with FMA (the reference's EPYC and this project's hosts).  Both are the
+// published table-driven methods of Szabolcs Nagy (ARM optimized-routines,
+// also glibc sysdeps/ieee754/flt-32/e_expf.c, e_powf.c):
+//   expf: x*32/ln2 = k + r, exp(x) = 2^(k/32) * P3(r), evaluated in double;
+//   powf: log2(x) from a 16-interval table (c_i, log2 c_i) and a degree-5
+//         polynomial in z/c_i - 1, times y, then the same exp2 kernel.
+// Reimplemented here from that description:
+//   * the exp2 table holds the bits of 2^(i/32) rounded to double, minus
+//     i<<47, computed from the definition;
+//   * the log2 table pairs invc_i ~ 1/(centre of interval i) with
+//     logc_i = RN(-log2(invc_i)) (each pair checked against that identity);
+//   * polynomial coefficients are the method's published minimax values;
+//   * every a*b+c of the FMA build is an explicit fma().
+// Agreement with glibc is checked on the host exhaustively (expf: every float
+// in [-104, 0]; powf: every positive float for several exponents) and on the
+// GPU against glibc values (tests/test_libm_glibc.py, tests/test_gpu_graph.py).
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+// In HIP builds these functions are device code reading their tables from
+// constant memory; compiled by a host compiler (the exhaustive checker in
+// tests/) the same source is plain host code.
+#ifdef __HIPCC__
+#define UMAP_HD __device__
+#define UMAP_LIBM_TABLE __constant__
+#else
+#define UMAP_HD
+#define UMAP_LIBM_TABLE
+#include <math.h>
+#endif
+
+namespace umapgpu {
+
+UMAP_HD inline double libm_bits_to_double(uint64_t u) {
+  double d;
+  memcpy(&d, &u, sizeof d);
+  return d;
+}
+UMAP_HD inline uint64_t libm_double_to_bits(double d) {
+  uint64_t u;
+  memcpy(&u, &d, sizeof u);
+  return u;
+}
+
+// bits(2^(i/32)) - (i << 47), i = 0..31
+UMAP_LIBM_TABLE static const uint64_t kLibmExp2Table[32] = {
+    0x3ff0000000000000ULL, 0x3fefd9b0d3158574ULL, 0x3fefb5586cf9890fULL, 0x3fef9301d0125b51ULL,
+    0x3fef72b83c7d517bULL, 0x3fef54873168b9aaULL, 0x3fef387a6e756238ULL, 0x3fef1e9df51fdee1ULL,
+    0x3fef06fe0a31b715ULL, 0x3feef1a7373aa9cbULL, 0x3feedea64c123422ULL, 0x3feece086061892dULL,
+    0x3feebfdad5362a27ULL, 0x3feeb42b569d4f82ULL, 0x3feeab07dd485429ULL, 0x3feea47eb03a5585ULL,
+    0x3feea09e667f3bcdULL, 0x3fee9f75e8ec5f74ULL, 0x3feea11473eb0187ULL, 0x3feea589994cce13ULL,
+    0x3feeace5422aa0dbULL, 0x3feeb737b0cdc5e5ULL, 0x3feec49182a3f090ULL, 0x3feed503b23e255dULL,
+    0x3feee89f995ad3adULL, 0x3feeff76f2fb5e47ULL, 0x3fef199bdd85529cULL, 0x3fef3720dcef9069ULL,
+    0x3fef5818dcfba487ULL, 0x3fef7c97337b9b5fULL, 0x3fefa4afa2a490daULL, 0x3fefd0765b6e4540ULL,
+};
+
+UMAP_HD inline uint64_t libm_exp2_table(uint32_t i) { return kLibmExp2Table[i & 31u]; }
+
+UMAP_HD inline float libm_expf(float x) {
+  // Special cases: glibc returns 0 below log(2^-150), inf above log(2^128),
+  // and x+x for NaN (+inf stays +inf, -inf gives 0).
+  if (!(x == x)) return x + x;
+  if (x < -0x1.9fe368p6f) return 0.0f;
+  if (x > 0x1.62e42ep6f) return (float)INFINITY;
+  const double N = 32.0;
+  const double inv_ln2_n = 0x1.71547652b82fep+0 * N;
+  const double shift = 0x1.8p+52;
+  const double c0 = 0x1.c6af84b912394p-5 / N / N / N;
+  const double c1 = 0x1.ebfce50fac4f3p-3 / N / N;
+  const double c2 = 0x1.62e42ff0c52d6p-1 / N;
+  double xd = (double)x;
+  double kd = fma(inv_ln2_n, xd, shift);
+  uint64_t ki = libm_double_to_bits(kd);
+  kd -= shift;
+  double r = fma(inv_ln2_n, xd, -kd);
+  uint64_t t = libm_exp2_table((uint32_t)(ki % 32u));
+  t += ki << (52 - 5);
+  double s = libm_bits_to_double(t);
+  double z = fma(c0, r, c1);
+  double r2 = r * r;
+  double y = fma(c2, r, 1.0);
+  y = fma(z, r2, y);
+  y = y * s;
+  return (float)y;
+}
+
+// log2 table of the powf method: {invc_i, logc_i} for the 16 subintervals of
+// [0x3f330000, 2*0x3f330000) in float bits.
+UMAP_LIBM_TABLE static const double kLibmLog2Table[16][2] = {
+    {0x1.661ec79f8f3bep+0, -0x1.efec65b963019p-2}, {0x1.571ed4aaf883dp+0, -0x1.b0b6832d4fca4p-2},
+    {0x1.49539f0f010bp+0, -0x1.7418b0a1fb77bp-2},  {0x1.3c995b0b80385p+0, -0x1.39de91a6dcf7bp-2},
+    {0x1.30d190c8864a5p+0, -0x1.01d9bf3f2b631p-2}, {0x1.25e227b0b8eap+0, -0x1.97c1d1b3b7afp-3},
+    {0x1.1bb4a4a1a343fp+0, -0x1.2f9e393af3c9fp-3}, {0x1.12358f08ae5bap+0, -0x1.960cbbf788d5cp-4},
+    {0x1.0953f419900a7p+0, -0x1.a6f9db6475fcep-5}, {0x1p+0, 0.0},
+    {0x1.e608cfd9a47acp-1, 0x1.338ca9f24f53dp-4},  {0x1.ca4b31f026aap-1, 0x1.476a9543891bap-3},
+    {0x1.b2036576afce6p-1, 0x1.e840b4ac4e4d2p-3},  {0x1.9c2d163a1aa2dp-1, 0x1.40645f0c6651cp-2},
+    {0x1.886e6037841edp-1, 0x1.88e9c2c1b9ff8p-2},  {0x1.767dcf5534862p-1, 0x1.ce0a44eb17bccp-2},
+};
+
+UMAP_HD inline void libm_log2_table(uint32_t i, double* invc, double* logc) {
+  *invc = kLibmLog2Table[i & 15u][0];
+  *logc = kLibmLog2Table[i & 15u][1];
+}
+
+UMAP_HD inline uint32_t libm_float_to_bits(float f) {
+  uint32_t u;
+  memcpy(&u, &f, sizeof u);
+  return u;
+}
+UMAP_HD inline float libm_bits_to_float(uint32_t u) {
+  float f;
+  memcpy(&f, &u, sizeof f);
+  return f;
+}
+
+// log2(x) for the normalised bit pattern ix (biased exponent may be < 1).
+UMAP_HD inline double libm_log2_inline(uint32_t ix) {
+  const uint32_t off = 0x3f330000u;
+  const uint32_t tmp = ix - off;
+  const uint32_t i = (tmp >> (23 - 4)) % 16u;
+  const uint32_t top = tmp & 0xff800000u;
+  const uint32_t iz = ix - top;
+  const int32_t k = (int32_t)top >> 23;
+  double invc, logc;
+  libm_log2_table(i, &invc, &logc);
+  const double z = (double)libm_bits_to_float(iz);
+  const double a0 = 0x1.27616c9496e0bp-2, a1 = -0x1.71969a075c67ap-2, a2 = 0x1.ec70a6ca7baddp-2,
+               a3 = -0x1.7154748bef6c8p-1, a4 = 0x1.71547652ab82bp0;
+  const double r = fma(z, invc, -1.0);
+  const double y0 = logc + (double)k;
+  const double r2 = r * r;
+  double y = fma(a0, r, a1);
+  const double p = fma(a2, r, a3);
+  const double r4 = r2 * r2;
+  double q = fma(a4, r, y0);
+  q = fma(p, r2, q);
+  y = fma(y, r4, q);
+  return y;
+}
+
+UMAP_HD inline float libm_exp2_inline(double xd, uint32_t sign_bias) {
+  const double shift = 0x1.8p+52 / 32.0;
+  const double c0 = 0x1.c6af84b912394p-5, c1 = 0x1.ebfce50fac4f3p-3, c2 = 0x1.62e42ff0c52d6p-1;
+  double kd = xd + shift;
+  const uint64_t ki = libm_double_to_bits(kd);
+  kd -= shift;
+  const double r = xd - kd;
+  uint64_t t = libm_exp2_table((uint32_t)(ki % 32u));
+  const uint64_t ski = ki + sign_bias;
+  t += ski << (52 - 5);
+  const double s = libm_bits_to_double(t);
+  const double z = fma(c0, r, c1);
+  const double r2 = r * r;
+  double y = fma(c2, r, 1.0);
+  y = fma(z, r2, y);
+  y = y * s;
+  return (float)y;
+}
+
+// 0: not an integer, 1: odd integer, 2: even integer (iy finite, non-zero).
+UMAP_HD inline int libm_checkint(uint32_t iy) {
+  const int e = (int)(iy >> 23 & 0xff);
+  if (e < 0x7f) return 0;
+  if (e > 0x7f + 23) return 2;
+  if (iy & ((1u << (0x7f + 23 - e)) - 1)) return 0;
+  if (iy & (1u << (0x7f + 23 - e))) return 1;
+  return 2;
+}
+
+UMAP_HD inline bool libm_zeroinfnan(uint32_t ix) { return 2u * ix - 1u >= 2u * 0x7f800000u - 1u; }
+
+UMAP_HD inline float libm_powf(float x, float y) {
+  uint32_t sign_bias = 0;
+  uint32_t ix = libm_float_to_bits(x);
+  const uint32_t iy = libm_float_to_bits(y);
+  if (ix - 0x00800000u >= 0x7f800000u - 0x00800000u || libm_zeroinfnan(iy)) {
+    if (libm_zeroinfnan(iy)) {
+      if (2u * iy == 0u) return 1.0f;
+      if (ix == 0x3f800000u) return 1.0f;
+      if (2u * ix > 2u * 0x7f800000u || 2u * iy > 2u * 0x7f800000u) return x + y;
+      if (2u * ix == 2u * 0x3f800000u) return 1.0f;
+      if ((2u * ix < 2u * 0x3f800000u) == !(iy & 0x80000000u)) return 0.0f;
+      return y * y;
+    }
+    if (libm_zeroinfnan(ix)) {
+      float x2 = x * x;
+      if ((ix & 0x80000000u) && libm_checkint(iy) == 1) x2 = -x2;
+      return (iy & 0x80000000u) ? 1.0f / x2 : x2;
+    }
+    if (ix & 0x80000000u) {
+      const int yint = libm_checkint(iy);
+      if (yint == 0) return (x - x) / (x - x);  // NaN
+      if (yint == 1) sign_bias = 1u << (5 + 11);
+      ix &= 0x7fffffffu;
+    }
+    if (ix < 0x00800000u) {  // subnormal: normalise, the exponent goes negative
+      ix = libm_float_to_bits(libm_bits_to_float(ix) * 0x1p23f);
+      ix &= 0x7fffffffu;
+      ix -= 23u << 23;
+    }
+  }
+  const double logx = libm_log2_inline(ix);
+  const double ylogx = (double)y * logx;
+  if ((libm_double_to_bits(ylogx) >> 47 & 0xffffu) >= libm_double_to_bits(126.0) >> 47) {
+    if (ylogx > 0x1.fffffffd1d571p+6) return sign_bias ? -(float)INFINITY : (float)INFINITY;
+    if (ylogx <= -150.0) return sign_bias ? -0.0f : 0.0f;
+  }
+  return libm_exp2_inline(ylogx, sign_bias);
+}
+
+}  // namespace umapgpu

@@ -19,9 +19,12 @@
 // together.  The tail snapshot is shared by all lanes: its attractive
 // updates are f32 hardware atomics (or plain racy stores, as the reference).
 // The f64 schedule recurrence is evaluated exactly as the reference does, so
-// every positive update and every negative-sample count is identical.
+// every positive update and every negative-sample count is identical, and
+// powf is the glibc-equivalent libm_powf, so the serial mode reproduces the
+// sequential CPU run bit for bit.
 #include <hip/hip_runtime.h>
 
+#include "libm_glibc.hpp"
 #include "optimize.hpp"
 #include "primitives.hpp"
 
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void sgd_rows_kernel(float* __restrict__ H, fl
       }
       float gc = 0.0f;
       if (d2 > 0.0f) {
-        const float pb = powf(d2, A.b);
+        const float pb = libm_powf(d2, A.b);
         gc = pos ? attract_coeff(A, d2, pb) : repel_coeff(A, d2, pb);
       }
       if (pos || gc > 0.0f) {
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H
         d2 = d2 + df * df;
       }
       float gc = 0.0f;
-      if (d2 > 0.0f) gc = attract_coeff(A, d2, powf(d2, A.b));
+      if (d2 > 0.0f) gc = attract_coeff(A, d2, libm_powf(d2, A.b));
       for (uint32_t d = 0; d < dim; ++d) {
         const float tv = t[d];
         const float ga = clip4(gc * (h[d] - tv)) * A.alpha;
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H
           dd = dd + df * df;
         }
         float g2 = 0.0f;
-        if (dd > 0.0f) g2 = repel_coeff(A, dd, powf(dd, A.b));
+        if (dd > 0.0f) g2 = repel_coeff(A, dd, libm_powf(dd, A.b));
         if (g2 > 0.0f)
           for (uint32_t d = 0; d < dim; ++d) h[d] = h[d] + clip4(g2 * (h[d] - o[d])) * A.alpha;
       }
@@ -321,7 +324,7 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
         d2 = d2 + df * df;
       }
       float gc = 0.0f;
-      if (d2 > 0.0f) gc = attract_coeff(A, d2, powf(d2, A.b));
+      if (d2 > 0.0f) gc = attract_coeff(A, d2, libm_powf(d2, A.b));
       for (uint32_t d = 0; d < dim; ++d) {
         const float df = h[d] - t[d];
         const float g = clip4(gc * df);
@@ -342,7 +345,7 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
           dd = dd + df * df;
         }
         float g2 = 0.0f;
-        if (dd > 0.0f) g2 = repel_coeff(A, dd, powf(dd, A.b));
+        if (dd > 0.0f) g2 = repel_coeff(A, dd, libm_powf(dd, A.b));
         if (g2 > 0.0f)
           for (uint32_t d = 0; d < dim; ++d) {
             const float df = h[d] - o[d];

@@ -122,6 +122,7 @@ def lib():
             "umapgpu_knn_bruteforce_device": ([P, u64, u64, u64, P, P], i32),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
+            "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -726,6 +727,14 @@ def debug_copy(src: DeviceArray, dst: DeviceArray, nbytes: int) -> None:
 
 def synchronize() -> None:
     _check(lib().umapgpu_synchronize())
+
+
+def debug_powf(x: np.ndarray, e: float) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    dx = DeviceArray.from_host(x)
+    dy = DeviceArray(x.shape, np.float32)
+    _check(lib().umapgpu_debug_powf_device(P(dx.ptr), e, P(dy.ptr), x.size))
+    return dy.to_host()
 
 
 def debug_expf(x: np.ndarray) -> np.ndarray:
