@@ -64,7 +64,7 @@ def main() -> int:
     ap.add_argument("--dim", type=int, default=64)
     ap.add_argument("--k", type=int, default=15)
     ap.add_argument("--epochs", type=int, default=500)
-    ap.add_argument("--mode", choices=["hogwild", "hogwild_racy"], default="hogwild")
+    ap.add_argument("--mode", choices=["hogwild", "hogwild_racy", "hogwild_rows"], default="hogwild")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's SGD sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-quality", action="store_true")
@@ -122,7 +122,8 @@ def main() -> int:
         src.free()
         dst.free()
 
-    mode = U.SGD_HOGWILD if args.mode == "hogwild" else U.SGD_HOGWILD_RACY
+    mode = {"hogwild": U.SGD_HOGWILD, "hogwild_racy": U.SGD_HOGWILD_RACY,
+            "hogwild_rows": U.SGD_HOGWILD_ROWS}[args.mode]
     times, kernel_ms, updates, negs, prep_ms = [], [], [], [], []
     last_opt = None
     for it in range(args.warmup + args.steps):
@@ -161,7 +162,9 @@ def main() -> int:
     achieved = bpu * float(sum(updates)) / kernel_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "sgd_rows_kernel<2,u16,atomic>" if args.mode == "hogwild" else "sgd_rows_kernel<2,u16,racy>",
+                "kernel": {"hogwild": "sgd_edges_kernel<2,u16,atomic tail>",
+                           "hogwild_racy": "sgd_edges_kernel<2,u16,racy tail>",
+                           "hogwild_rows": "sgd_rows_kernel<2,u16,atomic tail>"}[args.mode],
                 "bytes_per_update": bpu, "avg_launch_us": round(1e6 * kernel_s / (epochs * len(kernel_ms)), 2),
                 "updates_per_launch": round(float(sum(updates)) / (epochs * len(kernel_ms)), 1)}
     traffic_file = os.path.join(REPO, "profiles", "sgd_traffic_c2.json")
@@ -211,25 +214,21 @@ def main() -> int:
         om = O.Manifold(st["indptr"], st["indices"], st["data"], st["sigmas"], st["rhos"], n, st["a"], st["b"])
         ost = O.OptimizerState(om, init, epochs, seed=2)
         threads = O.num_threads()
-        # time one epoch, then run as many as fit the bound (epochs 1.. of 500;
-        # epoch 0 samples nothing)
+        # epochs 1.. of the same 500-epoch schedule in chunks of 5 until the
+        # time bound (epoch 0 samples nothing)
         ost.step_epochs(1, "hogwild")
-        done = 1
-        t = time.perf_counter()
-        ost.step_epochs(1, "hogwild")
-        one = time.perf_counter() - t
-        done += 1
-        more = int(max(0, min(epochs - done, args.cpu_seconds / max(one, 1e-6) - 1)))
         c0 = int(ost.counts[0])
         t = time.perf_counter()
-        if more:
-            ost.step_epochs(more, "hogwild")
+        done = 0
+        while done + 1 < epochs and time.perf_counter() - t < args.cpu_seconds:
+            chunk = min(5, epochs - 1 - done)
+            ost.step_epochs(chunk, "hogwild")
+            done += chunk
         dt = time.perf_counter() - t
-        cpu_updates = int(ost.counts[0]) - c0
-        cpu_rate = cpu_updates / dt if more else float(ost.counts[0]) / one
+        cpu_rate = (int(ost.counts[0]) - c0) / dt
         out["cpu_baseline"] = {"value": round(cpu_rate, 1), "unit": "edge-updates/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"same graph and init, Hogwild epochs {done}..{done + more - 1} of {epochs} "
+                               "sample": f"same graph and init, Hogwild epochs 1..{done} of {epochs} "
                                          f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
         log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")
 

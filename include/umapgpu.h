@@ -46,14 +46,20 @@ typedef enum umapgpu_metric_type { UMAPGPU_METRIC_EUCLIDEAN = 0, UMAPGPU_METRIC_
 
 /* SGD execution modes of Optimizer::step_epochs (an addition: the reference
  * always runs its Rayon Hogwild loop, optimizer.rs:337).
- *   HOGWILD: one lane per head row walks its edges in order; tail-snapshot
- *            updates use hardware f32 atomics (no lost updates).
+ *   HOGWILD: edge-parallel; each epoch compacts its active edges, one lane
+ *            runs one edge's positive and negative samples in order and adds
+ *            its change to the head row with f32 atomics; tail-snapshot
+ *            updates are f32 atomics too (no lost updates).  Default.
  *   HOGWILD_RACY: as HOGWILD with plain racy tail stores, as the reference.
- *   SERIAL: the reference's edge order on one lane (deterministic).       */
+ *   SERIAL: the reference's edge order on one lane (deterministic,
+ *           bit-identical to a sequential CPU run).
+ *   HOGWILD_ROWS: one lane per head row walks its edges in order with the
+ *            row in registers (the reference's per-thread ordering). */
 typedef enum umapgpu_sgd_mode {
   UMAPGPU_SGD_HOGWILD = 0,
   UMAPGPU_SGD_SERIAL = 1,
-  UMAPGPU_SGD_HOGWILD_RACY = 2
+  UMAPGPU_SGD_HOGWILD_RACY = 2,
+  UMAPGPU_SGD_HOGWILD_ROWS = 3
 } umapgpu_sgd_mode;
 
 /* UmapConfig (config.rs:206-235) with ManifoldParams (:9-52), GraphParams

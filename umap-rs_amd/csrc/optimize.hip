@@ -363,6 +363,159 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
   stats[1] += nneg;
 }
 
+// Edge-parallel epoch, step 1: append the edges active this epoch
+// (next[e] <= epoch) to a list, one atomic per wave; block 0 also clears the
+// counter the following epoch will use (the two alternate).
+template <typename NextT>
+__global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restrict__ next, uint64_t n_edges, NextT ep,
+                                                           uint32_t* __restrict__ list,
+                                                           uint32_t* __restrict__ counters, uint32_t slot) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[slot ^ 1u] = 0u;
+  const bool act = e < n_edges && next[e] <= ep;
+  const uint64_t m = __ballot(act);
+  if (m == 0ull) return;  // wave-uniform
+  const int lane = threadIdx.x & 63;
+  const uint32_t before = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&counters[slot], (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (act) list[base + before] = (uint32_t)e;
+}
+
+// Edge-parallel epoch, step 2: one lane per active edge (persistent grid).
+// The lane runs the edge's positive sample and its negative samples in the
+// reference's order on a private copy of H[j], then adds the change to H[j]
+// with f32 atomics (edges of one row run side by side here, so a plain
+// store would drop the others' updates).  Tail-snapshot updates are atomic
+// (ATOMIC_TAIL) or plain racy stores as in the reference.  Loads of the
+// edge's schedule state are coalesced: active lanes read neighbouring edges.
+template <int D, typename NextT, bool ATOMIC_TAIL>
+__global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
+                                                        const uint32_t* __restrict__ head,
+                                                        const uint32_t* __restrict__ tail,
+                                                        const double2* __restrict__ rate, double2* __restrict__ sched,
+                                                        NextT* __restrict__ next, const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ counters, uint32_t slot,
+                                                        SgdArgs A, unsigned long long* __restrict__ stats) {
+  const uint64_t count = counters[slot];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t npos = 0, nneg = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint32_t e = list[i];
+    const uint32_t j = head[e], k = tail[e];
+    float h[D], h0[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) h0[d] = h[d] = H[(uint64_t)j * D + d];
+    {  // attractive sample (optimize_layout_euclidean.rs:278-306)
+      float tv[D], diff[D];
+      float d2 = 0.0f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        tv[d] = T[(uint64_t)k * D + d];
+        diff[d] = h[d] - tv[d];
+        d2 = d2 + diff[d] * diff[d];
+      }
+      const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const float ga = clip4(gc * diff[d]) * A.alpha;
+        h[d] = h[d] + ga;
+        float* tp = T + (uint64_t)k * D + d;
+        if (ATOMIC_TAIL)
+          __hip_atomic_fetch_add(tp, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          *tp = tv[d] - ga;
+      }
+    }
+    double2 sc = sched[e];
+    const double2 rt = rate[e];
+    sc.x = sc.x + rt.x;                                             // :308
+    const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);     // :310-311
+    sc.y = sc.y + (double)nn * rt.y;                                // :343
+    sched[e] = sc;
+    next[e] = next_epoch<NextT>(sc.x);
+    const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+    for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
+      const uint32_t kk = rng_index(ekey, p, A.n);
+      if (kk == j) continue;
+      float diff[D];
+      float d2 = 0.0f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        diff[d] = h[d] - T[(uint64_t)kk * D + d];
+        d2 = d2 + diff[d] * diff[d];
+      }
+      const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+      if (g2 > 0.0f) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * A.alpha;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      __hip_atomic_fetch_add(H + (uint64_t)j * D + d, h[d] - h0[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ++npos;
+    nneg += nn;
+  }
+  wave_add_stats(npos, nneg, stats);
+}
+
+__global__ void fill_head_kernel(const uint64_t* __restrict__ row_ptr, uint64_t n, uint32_t* __restrict__ head) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (uint64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) head[e] = (uint32_t)r;
+}
+
+unsigned persistent_blocks() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    UMAP_HIP(hipGetDevice(&dev));
+    UMAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) cus = 256;
+  }
+  return (unsigned)cus * 8u;
+}
+
+template <typename NextT, bool ATOMIC_TAIL>
+void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  const uint32_t slot = (uint32_t)(A.epoch & 1u);
+  active_edges_kernel<NextT><<<grid_for(st->n_edges, 256), 256, 0, s>>>(
+      st->next.as<NextT>(), st->n_edges, (NextT)A.epoch, st->active.as<uint32_t>(), st->counters.as<uint32_t>(), slot);
+  UMAP_LAUNCH_CHECK();
+  const unsigned grid = std::min<unsigned>(persistent_blocks(), grid_for(st->n_edges, 256));
+  float* H = st->head_emb.as<float>();
+  float* T = st->tail_emb.as<float>();
+  const uint32_t* hd = st->head.as<uint32_t>();
+  const uint32_t* tl = st->tail.as<uint32_t>();
+  const double2* rt = st->rate.as<double2>();
+  double2* sc = st->sched.as<double2>();
+  NextT* nx = st->next.as<NextT>();
+  const uint32_t* li = st->active.as<uint32_t>();
+  const uint32_t* ct = st->counters.as<uint32_t>();
+  unsigned long long* stt = st->stats.as<unsigned long long>();
+#define UMAP_EDGES_CASE(DD)                                                                                  \
+  case DD:                                                                                                   \
+    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot, A, stt); \
+    break;
+  switch (st->dim) {
+    UMAP_EDGES_CASE(1)
+    UMAP_EDGES_CASE(2)
+    UMAP_EDGES_CASE(3)
+    UMAP_EDGES_CASE(4)
+    UMAP_EDGES_CASE(5)
+    UMAP_EDGES_CASE(6)
+    UMAP_EDGES_CASE(7)
+    UMAP_EDGES_CASE(8)
+    default:
+      throw Error(Err::Unsupported, "edge-parallel SGD supports n_components <= 8");
+  }
+#undef UMAP_EDGES_CASE
+  UMAP_LAUNCH_CHECK();
+}
+
 template <typename NextT, bool ATOMIC>
 void launch_rows(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const unsigned grid = grid_for(st->n, 256);
@@ -406,11 +559,20 @@ void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
                                                 st->stats.as<unsigned long long>());
       UMAP_LAUNCH_CHECK();
       break;
+    case SgdMode::HogwildRows:
+      launch_rows<NextT, true>(st, A, s);
+      break;
     case SgdMode::HogwildRacy:
-      launch_rows<NextT, false>(st, A, s);
+      if (st->dim <= 8)
+        launch_edges<NextT, false>(st, A, s);
+      else
+        launch_rows<NextT, false>(st, A, s);
       break;
     default:
-      launch_rows<NextT, true>(st, A, s);
+      if (st->dim <= 8)
+        launch_edges<NextT, true>(st, A, s);
+      else
+        launch_rows<NextT, true>(st, A, s);
   }
 }
 
@@ -435,6 +597,20 @@ void build_perm(OptimizerState* st, hipStream_t s) {
 void alloc_next(OptimizerState* st) {
   st->next16 = st->total_epochs <= 65535;
   st->next = DevBuf(std::max<uint64_t>(st->n_edges, 1) * (st->next16 ? 2 : 4));
+}
+
+// Per-edge head rows and the active-list buffers of the edge-parallel epoch.
+void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
+  const uint64_t E = st->n_edges;
+  if (E >= 0xffffffffull) throw Error(Err::Unsupported, "more than 2^32-1 SGD edges is not supported");
+  st->head = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  st->active = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  st->counters = DevBuf(8);
+  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, 8, s));
+  if (st->n) {
+    fill_head_kernel<<<grid_for(st->n, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), st->n, st->head.as<uint32_t>());
+    UMAP_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace
@@ -526,6 +702,7 @@ void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t di
     UMAP_HIP(hipStreamSynchronize(s));
   }
   build_perm(st, s);
+  alloc_edge_parallel(st, s);
   st->stats = DevBuf(16);
   UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
   UMAP_HIP(hipEventRecord(ev1, s));
@@ -582,6 +759,7 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
   if (n * dim) UMAP_HIP(hipMemcpyAsync(st->head_emb.p, embedding, n * dim * 4, hipMemcpyHostToDevice, s));
   UMAP_HIP(hipStreamSynchronize(s));
   build_perm(st, s);
+  alloc_edge_parallel(st, s);
   st->stats = DevBuf(16);
   UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
   UMAP_HIP(hipStreamSynchronize(s));
@@ -591,6 +769,9 @@ void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
   const uint64_t start = st->current_epoch, end = st->current_epoch + n_epochs;
   if (st->n * st->dim)
     UMAP_HIP(hipMemcpyAsync(st->tail_emb.p, st->head_emb.p, st->n * st->dim * 4, hipMemcpyDeviceToDevice, s));
+  // active-list counters start at zero; within the call each epoch's list
+  // kernel clears the counter of the epoch after it
+  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, 8, s));
   hipEvent_t ev0, ev1;
   UMAP_HIP(hipEventCreate(&ev0));
   UMAP_HIP(hipEventCreate(&ev1));

@@ -4,7 +4,7 @@
 
 namespace umapgpu {
 
-enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2 };
+enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2, HogwildRows = 3 };
 
 // Device-resident Optimizer (optimizer.rs:28-56).  Edges are kept in the
 // reference's CSR row order, so an edge's head is its row in `row_ptr`.
@@ -25,7 +25,10 @@ struct OptimizerState {
   DevBuf rate;      // double2[E]: {epochs_per_sample, epochs_per_negative_sample}
   DevBuf sched;     // double2[E]: {epoch_of_next_sample, epoch_of_next_negative_sample}
   DevBuf next;      // u16/u32[E]: ceil(epoch_of_next_sample), the activity test
-  DevBuf perm;      // u32[n]: rows by descending degree (lane load balance)
+  DevBuf perm;      // u32[n]: rows by descending degree (row-parallel mode)
+  DevBuf head;      // u32[E]: head row of each edge (edge-parallel mode)
+  DevBuf active;    // u32[E]: this epoch's active edges (edge-parallel mode)
+  DevBuf counters;  // u32[2]: active-list lengths, alternating by epoch parity
   DevBuf stats;     // u64[2]: positive updates, negative samples
 
   double last_step_ms = 0.0;

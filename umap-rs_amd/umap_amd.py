@@ -34,7 +34,7 @@ P = C.c_void_p
 
 OK, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, OVERSTEP, NO_DEVICE, UNSUPPORTED = range(7)
 
-SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY = 0, 1, 2
+SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY, SGD_HOGWILD_ROWS = 0, 1, 2, 3
 
 
 class UmapPanic(RuntimeError):
