@@ -363,34 +363,53 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
   stats[1] += nneg;
 }
 
-// Edge-parallel epoch, step 1: append the edges active this epoch
-// (next[e] <= epoch) to a list, one atomic per wave; block 0 also clears the
-// counter the following epoch will use (the two alternate).
+// Edge-parallel epoch layout.  Active edges are appended to kShards list
+// regions; block b of the list kernel appends to region b % kShards, so each
+// region's counter sees one atomic per block and the counters sit on
+// separate 128-byte lines.  Region capacity is ceil(blocks / kShards) * 256.
+constexpr uint32_t kShards = 128;
+constexpr uint32_t kCounterStride = 32;  // u32 words between counters (128 B)
+constexpr uint32_t kBlocksPerShard = 16;  // SGD blocks walking one region
+constexpr int kPrefetch = 6;              // negative samples gathered up front
+
+// Step 1: list the edges active this epoch (next[e] <= epoch).  Block 0 also
+// clears the counters the following epoch will use (the two sets alternate).
 template <typename NextT>
 __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restrict__ next, uint64_t n_edges, NextT ep,
                                                            uint32_t* __restrict__ list,
-                                                           uint32_t* __restrict__ counters, uint32_t slot) {
+                                                           uint32_t* __restrict__ counters, uint32_t slot,
+                                                           uint64_t cap) {
+  __shared__ uint32_t wcount[4];
+  __shared__ uint32_t block_base;
+  const uint32_t shard = blockIdx.x % kShards;
+  if (blockIdx.x == 0 && threadIdx.x < kShards)
+    counters[((slot ^ 1u) * kShards + threadIdx.x) * kCounterStride] = 0u;
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[slot ^ 1u] = 0u;
   const bool act = e < n_edges && next[e] <= ep;
   const uint64_t m = __ballot(act);
-  if (m == 0ull) return;  // wave-uniform
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t before = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&counters[slot], (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (act) list[base + before] = (uint32_t)e;
+  if (lane == 0) wcount[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    block_base = tot ? atomicAdd(&counters[(slot * kShards + shard) * kCounterStride], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t woff = 0;
+  for (int q = 0; q < w; ++q) woff += wcount[q];
+  if (act) list[(uint64_t)shard * cap + block_base + woff + before] = (uint32_t)e;
 }
 
-// Edge-parallel epoch, step 2: one lane per active edge (persistent grid).
-// The lane runs the edge's positive sample and its negative samples in the
-// reference's order on a private copy of H[j], then adds the change to H[j]
-// with f32 atomics (edges of one row run side by side here, so a plain
-// store would drop the others' updates).  Tail-snapshot updates are atomic
-// (ATOMIC_TAIL) or plain racy stores as in the reference.  Loads of the
-// edge's schedule state are coalesced: active lanes read neighbouring edges.
+// Step 2: one lane per active edge.  The lane runs the edge's positive
+// sample and its negative samples in the reference's order on a private copy
+// of H[j]; the head changes of lanes sharing a head row are summed across
+// the wave (a segmented scan) and applied with one f32 atomic per row and
+// dimension.  Tail-snapshot updates are atomic (ATOMIC_TAIL) or plain racy
+// stores as in the reference.  Every gather of an edge -- H[j], T[tail] and
+// the first kPrefetch negative targets, whose indices depend only on the
+// counter RNG -- is issued before any of them is used, so the lane waits
+// for one memory round trip instead of one per sample.
 template <int D, typename NextT, bool ATOMIC_TAIL>
 __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
                                                         const uint32_t* __restrict__ head,
@@ -398,66 +417,125 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
                                                         const double2* __restrict__ rate, double2* __restrict__ sched,
                                                         NextT* __restrict__ next, const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ counters, uint32_t slot,
-                                                        SgdArgs A, unsigned long long* __restrict__ stats) {
-  const uint64_t count = counters[slot];
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+                                                        uint64_t cap, SgdArgs A, unsigned long long* __restrict__ stats) {
+  const uint32_t shard = blockIdx.x % kShards, sub = blockIdx.x / kShards;
+  const uint64_t count = counters[(slot * kShards + shard) * kCounterStride];
+  const uint32_t* lst = list + (uint64_t)shard * cap;
+  const uint64_t stride = (uint64_t)(gridDim.x / kShards) * blockDim.x;
+  const int lane = threadIdx.x & 63;
   uint64_t npos = 0, nneg = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    const uint32_t e = list[i];
-    const uint32_t j = head[e], k = tail[e];
-    float h[D], h0[D];
+  for (uint64_t base_i = (uint64_t)sub * blockDim.x; base_i < count; base_i += stride) {  // block-uniform
+    const uint64_t i = base_i + threadIdx.x;
+    uint32_t j = 0xffffffffu;
+    float delta[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) h0[d] = h[d] = H[(uint64_t)j * D + d];
-    {  // attractive sample (optimize_layout_euclidean.rs:278-306)
-      float tv[D], diff[D];
-      float d2 = 0.0f;
+    for (int d = 0; d < D; ++d) delta[d] = 0.0f;
+    if (i < count) {
+      const uint32_t e = lst[i];
+      j = head[e];
+      const uint32_t k = tail[e];
+      double2 sc = sched[e];
+      const double2 rt = rate[e];
+      sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
+      const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
+      sc.y = sc.y + (double)nn * rt.y;                             // :343
+      sched[e] = sc;
+      next[e] = next_epoch<NextT>(sc.x);
+      const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+      // gathers, all in flight together
+      uint32_t kk[kPrefetch];
+      float tn[kPrefetch][D];
+#pragma unroll
+      for (int q = 0; q < kPrefetch; ++q) {
+        kk[q] = rng_index(ekey, (uint64_t)q, A.n);
+        if ((uint32_t)q < nn) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) tn[q][d] = T[(uint64_t)kk[q] * D + d];
+        }
+      }
+      float h[D], tv[D], diff[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
+        h[d] = H[(uint64_t)j * D + d];
         tv[d] = T[(uint64_t)k * D + d];
-        diff[d] = h[d] - tv[d];
-        d2 = d2 + diff[d] * diff[d];
       }
-      const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const float ga = clip4(gc * diff[d]) * A.alpha;
-        h[d] = h[d] + ga;
-        float* tp = T + (uint64_t)k * D + d;
-        if (ATOMIC_TAIL)
-          __hip_atomic_fetch_add(tp, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-          *tp = tv[d] - ga;
+      for (int d = 0; d < D; ++d) delta[d] = h[d];
+      {  // attractive sample (:278-306)
+        float d2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          diff[d] = h[d] - tv[d];
+          d2 = d2 + diff[d] * diff[d];
+        }
+        const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float ga = clip4(gc * diff[d]) * A.alpha;
+          h[d] = h[d] + ga;
+          float* tp = T + (uint64_t)k * D + d;
+          if (ATOMIC_TAIL)
+            __hip_atomic_fetch_add(tp, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          else
+            *tp = tv[d] - ga;
+        }
+      }
+      for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
+        uint32_t kp;
+        float o[D];
+        if (p < (uint32_t)kPrefetch) {
+#pragma unroll
+          for (int q = 0; q < kPrefetch; ++q)
+            if ((uint32_t)q == p) {
+              kp = kk[q];
+#pragma unroll
+              for (int d = 0; d < D; ++d) o[d] = tn[q][d];
+            }
+        } else {
+          kp = rng_index(ekey, p, A.n);
+#pragma unroll
+          for (int d = 0; d < D; ++d) o[d] = T[(uint64_t)kp * D + d];
+        }
+        if (kp == j) continue;
+        float d2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          diff[d] = h[d] - o[d];
+          d2 = d2 + diff[d] * diff[d];
+        }
+        const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+        if (g2 > 0.0f) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * A.alpha;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = h[d] - delta[d];
+      ++npos;
+      nneg += nn;
+    }
+    // segmented inclusive scan of the deltas over runs of equal head rows
+    uint32_t f = (lane == 0 || __shfl_up(j, 1, 64) != j) ? 1u : 0u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      float up[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) up[d] = __shfl_up(delta[d], (unsigned)off, 64);
+      const uint32_t fu = __shfl_up(f, (unsigned)off, 64);
+      if (lane >= off) {
+        if (!f) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) delta[d] = up[d] + delta[d];
+        }
+        f |= fu;
       }
     }
-    double2 sc = sched[e];
-    const double2 rt = rate[e];
-    sc.x = sc.x + rt.x;                                             // :308
-    const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);     // :310-311
-    sc.y = sc.y + (double)nn * rt.y;                                // :343
-    sched[e] = sc;
-    next[e] = next_epoch<NextT>(sc.x);
-    const uint64_t ekey = rng_edge_key(A.epoch_key, e);
-    for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
-      const uint32_t kk = rng_index(ekey, p, A.n);
-      if (kk == j) continue;
-      float diff[D];
-      float d2 = 0.0f;
+    const bool run_tail = (lane == 63) || (__shfl_down(j, 1, 64) != j);
+    if (run_tail && j != 0xffffffffu) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        diff[d] = h[d] - T[(uint64_t)kk * D + d];
-        d2 = d2 + diff[d] * diff[d];
-      }
-      const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
-      if (g2 > 0.0f) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * A.alpha;
-      }
+      for (int d = 0; d < D; ++d)
+        __hip_atomic_fetch_add(H + (uint64_t)j * D + d, delta[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-      __hip_atomic_fetch_add(H + (uint64_t)j * D + d, h[d] - h0[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ++npos;
-    nneg += nn;
   }
   wave_add_stats(npos, nneg, stats);
 }
@@ -468,24 +546,17 @@ __global__ void fill_head_kernel(const uint64_t* __restrict__ row_ptr, uint64_t 
   for (uint64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) head[e] = (uint32_t)r;
 }
 
-unsigned persistent_blocks() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    UMAP_HIP(hipGetDevice(&dev));
-    UMAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (cus <= 0) cus = 256;
-  }
-  return (unsigned)cus * 8u;
-}
+uint64_t list_capacity(uint64_t n_edges) { return ceil_div(ceil_div(std::max<uint64_t>(n_edges, 1), 256), kShards) * 256; }
 
 template <typename NextT, bool ATOMIC_TAIL>
 void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const uint32_t slot = (uint32_t)(A.epoch & 1u);
+  const uint64_t cap = list_capacity(st->n_edges);
   active_edges_kernel<NextT><<<grid_for(st->n_edges, 256), 256, 0, s>>>(
-      st->next.as<NextT>(), st->n_edges, (NextT)A.epoch, st->active.as<uint32_t>(), st->counters.as<uint32_t>(), slot);
+      st->next.as<NextT>(), st->n_edges, (NextT)A.epoch, st->active.as<uint32_t>(), st->counters.as<uint32_t>(), slot,
+      cap);
   UMAP_LAUNCH_CHECK();
-  const unsigned grid = std::min<unsigned>(persistent_blocks(), grid_for(st->n_edges, 256));
+  const unsigned grid = kShards * kBlocksPerShard;
   float* H = st->head_emb.as<float>();
   float* T = st->tail_emb.as<float>();
   const uint32_t* hd = st->head.as<uint32_t>();
@@ -496,9 +567,10 @@ void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const uint32_t* li = st->active.as<uint32_t>();
   const uint32_t* ct = st->counters.as<uint32_t>();
   unsigned long long* stt = st->stats.as<unsigned long long>();
-#define UMAP_EDGES_CASE(DD)                                                                                  \
-  case DD:                                                                                                   \
-    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot, A, stt); \
+#define UMAP_EDGES_CASE(DD)                                                                                       \
+  case DD:                                                                                                        \
+    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot, cap, A, \
+                                                                  stt);                                           \
     break;
   switch (st->dim) {
     UMAP_EDGES_CASE(1)
@@ -604,9 +676,9 @@ void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
   const uint64_t E = st->n_edges;
   if (E >= 0xffffffffull) throw Error(Err::Unsupported, "more than 2^32-1 SGD edges is not supported");
   st->head = DevBuf(std::max<uint64_t>(E, 1) * 4);
-  st->active = DevBuf(std::max<uint64_t>(E, 1) * 4);
-  st->counters = DevBuf(8);
-  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, 8, s));
+  st->active = DevBuf(list_capacity(E) * kShards * 4);
+  st->counters = DevBuf(2 * kShards * kCounterStride * 4);
+  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, st->counters.bytes, s));
   if (st->n) {
     fill_head_kernel<<<grid_for(st->n, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), st->n, st->head.as<uint32_t>());
     UMAP_LAUNCH_CHECK();
@@ -771,7 +843,7 @@ void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
     UMAP_HIP(hipMemcpyAsync(st->tail_emb.p, st->head_emb.p, st->n * st->dim * 4, hipMemcpyDeviceToDevice, s));
   // active-list counters start at zero; within the call each epoch's list
   // kernel clears the counter of the epoch after it
-  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, 8, s));
+  UMAP_HIP(hipMemsetAsync(st->counters.p, 0, st->counters.bytes, s));
   hipEvent_t ev0, ev1;
   UMAP_HIP(hipEventCreate(&ev0));
   UMAP_HIP(hipEventCreate(&ev1));
