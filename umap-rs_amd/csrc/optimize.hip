@@ -364,9 +364,12 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
 }
 
 // Edge-parallel epoch layout.  Active edges are appended to kShards list
-// regions; block b of the list kernel appends to region b % kShards, so each
-// region's counter sees one atomic per block and the counters sit on
-// separate 128-byte lines.  Region capacity is ceil(blocks / kShards) * 256.
+// regions, one atomic per block on a region counter (counters sit on
+// separate 128-byte lines).  Workgroups are dealt to the 8 XCDs round-robin,
+// so SGD block b -- which walks region b % kShards -- runs on XCD b % 8.
+// The list kernel therefore sends the edges of the g-th eighth of the edge
+// array (hence of the head rows) to the regions r with r % 8 == g: each
+// XCD's L2 then only holds its eighth of the head embedding.
 constexpr uint32_t kShards = 128;
 constexpr uint32_t kCounterStride = 32;  // u32 words between counters (128 B)
 constexpr uint32_t kBlocksPerShard = 16;  // SGD blocks walking one region
@@ -374,6 +377,8 @@ constexpr int kPrefetch = 6;              // negative samples gathered up front
 
 // Step 1: list the edges active this epoch (next[e] <= epoch).  Block 0 also
 // clears the counters the following epoch will use (the two sets alternate).
+constexpr uint32_t kXcds = 8;
+
 template <typename NextT>
 __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restrict__ next, uint64_t n_edges, NextT ep,
                                                            uint32_t* __restrict__ list,
@@ -381,11 +386,12 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
                                                            uint64_t cap) {
   __shared__ uint32_t wcount[4];
   __shared__ uint32_t block_base;
-  const uint32_t shard = blockIdx.x % kShards;
+  const uint32_t group = (uint32_t)(((uint64_t)blockIdx.x * kXcds) / gridDim.x);
+  const uint32_t shard = group + kXcds * (blockIdx.x % (kShards / kXcds));
   if (blockIdx.x == 0 && threadIdx.x < kShards)
     counters[((slot ^ 1u) * kShards + threadIdx.x) * kCounterStride] = 0u;
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = e < n_edges && next[e] <= ep;
+  const bool act = e < n_edges && __builtin_nontemporal_load(next + e) <= ep;
   const uint64_t m = __ballot(act);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t before = (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
@@ -398,7 +404,7 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
   __syncthreads();
   uint32_t woff = 0;
   for (int q = 0; q < w; ++q) woff += wcount[q];
-  if (act) list[(uint64_t)shard * cap + block_base + woff + before] = (uint32_t)e;
+  if (act) __builtin_nontemporal_store((uint32_t)e, list + (uint64_t)shard * cap + block_base + woff + before);
 }
 
 // Step 2: one lane per active edge.  The lane runs the edge's positive
@@ -431,26 +437,34 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
 #pragma unroll
     for (int d = 0; d < D; ++d) delta[d] = 0.0f;
     if (i < count) {
-      const uint32_t e = lst[i];
-      j = head[e];
-      const uint32_t k = tail[e];
-      double2 sc = sched[e];
-      const double2 rt = rate[e];
+      // per-edge state is streamed (non-temporal) so that it does not evict
+      // the embedding rows the gathers below reuse
+      const uint32_t e = __builtin_nontemporal_load(lst + i);
+      j = __builtin_nontemporal_load(head + e);
+      const uint32_t k = __builtin_nontemporal_load(tail + e);
+      double2 sc, rt;
+      sc.x = __builtin_nontemporal_load(&sched[e].x);
+      sc.y = __builtin_nontemporal_load(&sched[e].y);
+      rt.x = __builtin_nontemporal_load(&rate[e].x);
+      rt.y = __builtin_nontemporal_load(&rate[e].y);
       sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
       const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
       sc.y = sc.y + (double)nn * rt.y;                             // :343
-      sched[e] = sc;
-      next[e] = next_epoch<NextT>(sc.x);
+      __builtin_nontemporal_store(sc.x, &sched[e].x);
+      __builtin_nontemporal_store(sc.y, &sched[e].y);
+      __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
       const uint64_t ekey = rng_edge_key(A.epoch_key, e);
-      // gathers, all in flight together
-      uint32_t kk[kPrefetch];
+      // gathers, all in flight together; a draw equal to j is skipped later
+      // (the draw still counts, :315-317)
+      uint32_t self_mask = 0;
       float tn[kPrefetch][D];
 #pragma unroll
       for (int q = 0; q < kPrefetch; ++q) {
-        kk[q] = rng_index(ekey, (uint64_t)q, A.n);
+        const uint32_t kq = rng_index(ekey, (uint64_t)q, A.n);
+        self_mask |= (kq == j) ? (1u << q) : 0u;
         if ((uint32_t)q < nn) {
 #pragma unroll
-          for (int d = 0; d < D; ++d) tn[q][d] = T[(uint64_t)kk[q] * D + d];
+          for (int d = 0; d < D; ++d) tn[q][d] = T[(uint64_t)kq * D + d];
         }
       }
       float h[D], tv[D], diff[D];
@@ -481,22 +495,23 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
         }
       }
       for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
-        uint32_t kp;
+        bool skip;
         float o[D];
         if (p < (uint32_t)kPrefetch) {
+          skip = (self_mask >> p) & 1u;
 #pragma unroll
           for (int q = 0; q < kPrefetch; ++q)
             if ((uint32_t)q == p) {
-              kp = kk[q];
 #pragma unroll
               for (int d = 0; d < D; ++d) o[d] = tn[q][d];
             }
         } else {
-          kp = rng_index(ekey, p, A.n);
+          const uint32_t kp = rng_index(ekey, p, A.n);
+          skip = (kp == j);
 #pragma unroll
           for (int d = 0; d < D; ++d) o[d] = T[(uint64_t)kp * D + d];
         }
-        if (kp == j) continue;
+        if (skip) continue;
         float d2 = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -546,7 +561,12 @@ __global__ void fill_head_kernel(const uint64_t* __restrict__ row_ptr, uint64_t 
   for (uint64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) head[e] = (uint32_t)r;
 }
 
-uint64_t list_capacity(uint64_t n_edges) { return ceil_div(ceil_div(std::max<uint64_t>(n_edges, 1), 256), kShards) * 256; }
+// Region capacity: a region receives at most ceil(ceil(blocks/8)/16) + 1
+// list-kernel blocks of 256 edges.
+uint64_t list_capacity(uint64_t n_edges) {
+  const uint64_t blocks = ceil_div(std::max<uint64_t>(n_edges, 1), 256);
+  return (ceil_div(ceil_div(blocks, kXcds), kShards / kXcds) + 1) * 256;
+}
 
 template <typename NextT, bool ATOMIC_TAIL>
 void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
