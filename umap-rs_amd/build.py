@@ -23,8 +23,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-BUILD_DIR = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libumapgpu.so")
+BUILD_DIR = os.environ.get("UMAPGPU_BUILD_DIR") or os.path.join(HERE, "_build")
+LIB = os.environ.get("UMAPGPU_LIB_OUT") or os.path.join(HERE, "libumapgpu.so")
 SOURCES = ["api.hip", "graph.hip", "optimize.hip", "primitives.hip", "knn.hip"]
 ARCH = os.environ.get("UMAPGPU_ARCH", "gfx950")
 
@@ -37,7 +37,7 @@ COMMON = [
     "-Xarch_host", "-fno-builtin",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
     "-I" + os.path.join(REPO, "include"),
-]
+] + os.environ.get("UMAPGPU_EXTRA_FLAGS", "").split()
 
 
 def hipcc() -> str:

@@ -104,10 +104,15 @@ UMAP_LIBM_TABLE static const double kLibmLog2Table[16][2] = {
     {0x1.886e6037841edp-1, 0x1.88e9c2c1b9ff8p-2},  {0x1.767dcf5534862p-1, 0x1.ce0a44eb17bccp-2},
 };
 
-UMAP_HD inline void libm_log2_table(uint32_t i, double* invc, double* logc) {
-  *invc = kLibmLog2Table[i & 15u][0];
-  *logc = kLibmLog2Table[i & 15u][1];
-}
+// Where the two tables live for a call: constant memory by default; a
+// kernel may stage copies in LDS and pass those (lower latency for the
+// per-lane divergent lookups).
+struct LibmTables {
+  const uint64_t* exp2;  // 32 entries
+  const double* log2;    // 16 x {invc, logc}
+};
+
+UMAP_HD inline LibmTables libm_default_tables() { return LibmTables{kLibmExp2Table, &kLibmLog2Table[0][0]}; }
 
 UMAP_HD inline uint32_t libm_float_to_bits(float f) {
   uint32_t u;
@@ -121,15 +126,14 @@ UMAP_HD inline float libm_bits_to_float(uint32_t u) {
 }
 
 // log2(x) for the normalised bit pattern ix (biased exponent may be < 1).
-UMAP_HD inline double libm_log2_inline(uint32_t ix) {
+UMAP_HD inline double libm_log2_inline(uint32_t ix, const LibmTables& tb) {
   const uint32_t off = 0x3f330000u;
   const uint32_t tmp = ix - off;
   const uint32_t i = (tmp >> (23 - 4)) % 16u;
   const uint32_t top = tmp & 0xff800000u;
   const uint32_t iz = ix - top;
   const int32_t k = (int32_t)top >> 23;
-  double invc, logc;
-  libm_log2_table(i, &invc, &logc);
+  const double invc = tb.log2[2 * i], logc = tb.log2[2 * i + 1];
   const double z = (double)libm_bits_to_float(iz);
   const double a0 = 0x1.27616c9496e0bp-2, a1 = -0x1.71969a075c67ap-2, a2 = 0x1.ec70a6ca7baddp-2,
                a3 = -0x1.7154748bef6c8p-1, a4 = 0x1.71547652ab82bp0;
@@ -145,14 +149,14 @@ UMAP_HD inline double libm_log2_inline(uint32_t ix) {
   return y;
 }
 
-UMAP_HD inline float libm_exp2_inline(double xd, uint32_t sign_bias) {
+UMAP_HD inline float libm_exp2_inline(double xd, uint32_t sign_bias, const LibmTables& tb) {
   const double shift = 0x1.8p+52 / 32.0;
   const double c0 = 0x1.c6af84b912394p-5, c1 = 0x1.ebfce50fac4f3p-3, c2 = 0x1.62e42ff0c52d6p-1;
   double kd = xd + shift;
   const uint64_t ki = libm_double_to_bits(kd);
   kd -= shift;
   const double r = xd - kd;
-  uint64_t t = libm_exp2_table((uint32_t)(ki % 32u));
+  uint64_t t = tb.exp2[ki % 32u];
   const uint64_t ski = ki + sign_bias;
   t += ski << (52 - 5);
   const double s = libm_bits_to_double(t);
@@ -176,7 +180,7 @@ UMAP_HD inline int libm_checkint(uint32_t iy) {
 
 UMAP_HD inline bool libm_zeroinfnan(uint32_t ix) { return 2u * ix - 1u >= 2u * 0x7f800000u - 1u; }
 
-UMAP_HD inline float libm_powf(float x, float y) {
+UMAP_HD inline float libm_powf(float x, float y, const LibmTables& tb) {
   uint32_t sign_bias = 0;
   uint32_t ix = libm_float_to_bits(x);
   const uint32_t iy = libm_float_to_bits(y);
@@ -206,13 +210,15 @@ UMAP_HD inline float libm_powf(float x, float y) {
       ix -= 23u << 23;
     }
   }
-  const double logx = libm_log2_inline(ix);
+  const double logx = libm_log2_inline(ix, tb);
   const double ylogx = (double)y * logx;
   if ((libm_double_to_bits(ylogx) >> 47 & 0xffffu) >= libm_double_to_bits(126.0) >> 47) {
     if (ylogx > 0x1.fffffffd1d571p+6) return sign_bias ? -(float)INFINITY : (float)INFINITY;
     if (ylogx <= -150.0) return sign_bias ? -0.0f : 0.0f;
   }
-  return libm_exp2_inline(ylogx, sign_bias);
+  return libm_exp2_inline(ylogx, sign_bias, tb);
 }
+
+UMAP_HD inline float libm_powf(float x, float y) { return libm_powf(x, y, libm_default_tables()); }
 
 }  // namespace umapgpu

@@ -373,7 +373,41 @@ __global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, 
 constexpr uint32_t kShards = 128;
 constexpr uint32_t kCounterStride = 32;  // u32 words between counters (128 B)
 constexpr uint32_t kBlocksPerShard = 16;  // SGD blocks walking one region
-constexpr int kPrefetch = 6;              // negative samples gathered up front
+#ifndef UMAP_SGD_PREFETCH
+#define UMAP_SGD_PREFETCH 6
+#endif
+#ifndef UMAP_SGD_LDS_TABLES
+#define UMAP_SGD_LDS_TABLES 1
+#endif
+#ifndef UMAP_SGD_MIN_BLOCKS
+#define UMAP_SGD_MIN_BLOCKS 1
+#endif
+#ifndef UMAP_SGD_GATHER
+#define UMAP_SGD_GATHER 0
+#endif
+#ifndef UMAP_SGD_TAIL_CAS
+#define UMAP_SGD_TAIL_CAS 1
+#endif
+constexpr int kPrefetch = UMAP_SGD_PREFETCH;  // negative samples gathered up front
+
+// One row of the tail snapshot.  UMAP_SGD_GATHER selects the load's cache
+// policy: 0 default, 1 non-temporal, 2..4 buffer loads with sc0|sc1, sc1,
+// sc0 (the buffer's range check returns 0 past the end; rows are < n).
+template <int D>
+__device__ inline void load_trow(const float* __restrict__ T, __amdgpu_buffer_rsrc_t rsrc, uint32_t row, float* out) {
+#if UMAP_SGD_GATHER == 0
+#pragma unroll
+  for (int d = 0; d < D; ++d) out[d] = T[(uint64_t)row * D + d];
+#elif UMAP_SGD_GATHER == 1
+#pragma unroll
+  for (int d = 0; d < D; ++d) out[d] = __builtin_nontemporal_load(T + (uint64_t)row * D + d);
+#else
+  constexpr int aux = UMAP_SGD_GATHER == 2 ? 17 : (UMAP_SGD_GATHER == 3 ? 16 : 1);
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    out[d] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)((row * D + d) * 4u), 0, aux));
+#endif
+}
 
 // Step 1: list the edges active this epoch (next[e] <= epoch).  Block 0 also
 // clears the counters the following epoch will use (the two sets alternate).
@@ -417,16 +451,28 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
 // counter RNG -- is issued before any of them is used, so the lane waits
 // for one memory round trip instead of one per sample.
 template <int D, typename NextT, bool ATOMIC_TAIL>
-__global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
+__global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
                                                         const uint32_t* __restrict__ head,
                                                         const uint32_t* __restrict__ tail,
                                                         const double2* __restrict__ rate, double2* __restrict__ sched,
                                                         NextT* __restrict__ next, const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ counters, uint32_t slot,
                                                         uint64_t cap, SgdArgs A, unsigned long long* __restrict__ stats) {
+  // powf's two lookup tables, staged in LDS for low-latency divergent reads
+  __shared__ uint64_t s_exp2[32];
+  __shared__ double s_log2[32];
+  if (UMAP_SGD_LDS_TABLES && threadIdx.x < 32) {
+    const LibmTables c = libm_default_tables();
+    s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
+    s_log2[threadIdx.x] = c.log2[threadIdx.x];
+  }
+  __syncthreads();
+  const LibmTables tb = UMAP_SGD_LDS_TABLES ? LibmTables{s_exp2, s_log2} : libm_default_tables();
   const uint32_t shard = blockIdx.x % kShards, sub = blockIdx.x / kShards;
   const uint64_t count = counters[(slot * kShards + shard) * kCounterStride];
   const uint32_t* lst = list + (uint64_t)shard * cap;
+  const __amdgpu_buffer_rsrc_t trsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)T, (short)0, (int)(A.n * (uint32_t)D * 4u), 0x00020000);
   const uint64_t stride = (uint64_t)(gridDim.x / kShards) * blockDim.x;
   const int lane = threadIdx.x & 63;
   uint64_t npos = 0, nneg = 0;
@@ -462,17 +508,12 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
       for (int q = 0; q < kPrefetch; ++q) {
         const uint32_t kq = rng_index(ekey, (uint64_t)q, A.n);
         self_mask |= (kq == j) ? (1u << q) : 0u;
-        if ((uint32_t)q < nn) {
-#pragma unroll
-          for (int d = 0; d < D; ++d) tn[q][d] = T[(uint64_t)kq * D + d];
-        }
+        if ((uint32_t)q < nn) load_trow<D>(T, trsrc, kq, tn[q]);
       }
       float h[D], tv[D], diff[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        h[d] = H[(uint64_t)j * D + d];
-        tv[d] = T[(uint64_t)k * D + d];
-      }
+      for (int d = 0; d < D; ++d) h[d] = H[(uint64_t)j * D + d];
+      load_trow<D>(T, trsrc, k, tv);
 #pragma unroll
       for (int d = 0; d < D; ++d) delta[d] = h[d];
       {  // attractive sample (:278-306)
@@ -482,16 +523,36 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
           diff[d] = h[d] - tv[d];
           d2 = d2 + diff[d] * diff[d];
         }
-        const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+        const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b, tb)) : 0.0f;
+        float ga[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const float ga = clip4(gc * diff[d]) * A.alpha;
-          h[d] = h[d] + ga;
-          float* tp = T + (uint64_t)k * D + d;
-          if (ATOMIC_TAIL)
-            __hip_atomic_fetch_add(tp, -ga, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          else
-            *tp = tv[d] - ga;
+          ga[d] = clip4(gc * diff[d]) * A.alpha;
+          h[d] = h[d] + ga[d];
+        }
+        if (ATOMIC_TAIL && UMAP_SGD_TAIL_CAS && D == 2) {
+          // both coordinates in one 64-bit compare-and-swap
+          unsigned long long* tp = reinterpret_cast<unsigned long long*>(T + (uint64_t)k * D);
+          unsigned long long expect = ((unsigned long long)__float_as_uint(tv[D > 1 ? 1 : 0]) << 32) |
+                                      (unsigned long long)__float_as_uint(tv[0]);
+          for (;;) {
+            const float x0 = __uint_as_float((uint32_t)expect) - ga[0];
+            const float x1 = __uint_as_float((uint32_t)(expect >> 32)) - ga[D > 1 ? 1 : 0];
+            const unsigned long long want =
+                ((unsigned long long)__float_as_uint(x1) << 32) | (unsigned long long)__float_as_uint(x0);
+            const unsigned long long got = atomicCAS(tp, expect, want);
+            if (got == expect) break;
+            expect = got;
+          }
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            float* tp = T + (uint64_t)k * D + d;
+            if (ATOMIC_TAIL)
+              __hip_atomic_fetch_add(tp, -ga[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+              *tp = tv[d] - ga[d];
+          }
         }
       }
       for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
@@ -508,8 +569,7 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
         } else {
           const uint32_t kp = rng_index(ekey, p, A.n);
           skip = (kp == j);
-#pragma unroll
-          for (int d = 0; d < D; ++d) o[d] = T[(uint64_t)kp * D + d];
+          load_trow<D>(T, trsrc, kp, o);
         }
         if (skip) continue;
         float d2 = 0.0f;
@@ -518,7 +578,7 @@ __global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, f
           diff[d] = h[d] - o[d];
           d2 = d2 + diff[d] * diff[d];
         }
-        const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b)) : 0.0f;
+        const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b, tb)) : 0.0f;
         if (g2 > 0.0f) {
 #pragma unroll
           for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * A.alpha;

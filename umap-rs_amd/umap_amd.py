@@ -27,7 +27,7 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libumapgpu.so")
+LIB_PATH = os.environ.get("UMAPGPU_LIB") or os.path.join(_HERE, "libumapgpu.so")
 
 u64, u32, i32, f32, f64 = C.c_uint64, C.c_uint32, C.c_int32, C.c_float, C.c_double
 P = C.c_void_p
