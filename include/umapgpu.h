@@ -257,6 +257,46 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);
+/* ---- evaluators (no reference counterpart: the reference defines no quality
+ * measure).  Used to judge embeddings and to time wall-to-target-loss. ---- */
+typedef struct umapgpu_ce_terms {
+  double total;        /* attract + (n^2 - edges) / pairs * repulse_sum */
+  double attract;      /* sum over graph edges of w log(w/v) + (1-w) log((1-w)/(1-v)) */
+  double repulse_sum;  /* sum over the sampled pairs of -log(1-v) */
+  uint64_t edges;      /* graph edges with both rows finite */
+  uint64_t pairs;      /* sampled pairs kept (i != j, both rows finite) */
+} umapgpu_ce_terms;
+/* Fuzzy cross-entropy of an n x dim device embedding against a device CSR
+ * graph (indptr u64[n+1], indices u32, data f32), v = 1/(1+a|y_i-y_j|^(2b)),
+ * w and v clamped to [1e-12, 1-1e-12], f64 sums.  The n_random pairs are
+ * drawn by the counter RNG from `seed` (pair p = (index(2p), index(2p+1))). */
+UMAPGPU_API umapgpu_status umapgpu_fuzzy_cross_entropy_device(const uint64_t* indptr_device,
+                                                              const uint32_t* indices_device,
+                                                              const float* data_device, uint64_t n,
+                                                              const float* embedding_device, uint64_t dim, float a,
+                                                              float b, uint64_t n_random, uint64_t seed,
+                                                              umapgpu_ce_terms* out);
+/* The same against the manifold's (unfiltered, symmetric) graph and (a, b). */
+UMAPGPU_API umapgpu_status umapgpu_manifold_fuzzy_cross_entropy(const umapgpu_manifold* m,
+                                                                const float* embedding_device, uint64_t dim,
+                                                                uint64_t n_random, uint64_t seed,
+                                                                umapgpu_ce_terms* out);
+/* The same for the optimizer's current embedding and its manifold. */
+UMAPGPU_API umapgpu_status umapgpu_optimizer_fuzzy_cross_entropy(const umapgpu_optimizer* o, uint64_t n_random,
+                                                                 uint64_t seed, umapgpu_ce_terms* out);
+/* k nearest other rows of data (n x dim, dim in {1,2,3,4,8}) for each row
+ * listed in queries (device u32[nq]); out is nq x k, ties by lower index. */
+UMAPGPU_API umapgpu_status umapgpu_knn_query_device(const float* data_device, uint64_t n, uint64_t dim,
+                                                    const uint32_t* queries_device, uint64_t nq, uint64_t k,
+                                                    uint32_t* out_indices_device);
+/* Trustworthiness penalty sum_q sum_j max(rank(q, nbrs[q][j]) - k, 0), where
+ * rank is the position of a point in the ordering of all rows of data
+ * (n x dim) by (squared distance to q, index), q itself at position 0.
+ * trustworthiness = 1 - 2 / (nq k (2n - 3k - 1)) * penalty. */
+UMAPGPU_API umapgpu_status umapgpu_trustworthiness_penalty_device(const float* data_device, uint64_t n,
+                                                                  uint64_t dim, const uint32_t* queries_device,
+                                                                  uint64_t nq, const uint32_t* nbrs_device,
+                                                                  uint64_t k, double* penalty);
 /* Calibration hook: a plain streaming copy of `bytes` (a multiple of 16),
  * used to scale hardware traffic counters into bytes. */
 UMAPGPU_API umapgpu_status umapgpu_debug_copy_device(const void* src_device, void* dst_device, uint64_t bytes);

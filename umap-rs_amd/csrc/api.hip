@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/umapgpu.h"
+#include "evaluate.hpp"
 #include "graph.hpp"
 #include "knn.hpp"
 #include "optimize.hpp"
@@ -866,6 +867,69 @@ umapgpu_status umapgpu_knn_bruteforce_device(const float* data, uint64_t n, uint
   return guarded([&] {
     ensure_device();
     knn_bruteforce(data, n, dim, k, idx, dists, stream());
+  });
+}
+
+namespace {
+void ce_out(const CeTerms& t, umapgpu_ce_terms* out) {
+  out->total = t.total;
+  out->attract = t.attract;
+  out->repulse_sum = t.repulse_sum;
+  out->edges = t.edges;
+  out->pairs = t.pairs;
+}
+}  // namespace
+
+umapgpu_status umapgpu_fuzzy_cross_entropy_device(const uint64_t* indptr, const uint32_t* indices, const float* data,
+                                                  uint64_t n, const float* emb, uint64_t dim, float a, float b,
+                                                  uint64_t n_random, uint64_t seed, umapgpu_ce_terms* out) {
+  return guarded([&] {
+    ensure_device();
+    if (!out) throw Error(Err::InvalidArgument, "null output");
+    ce_out(fuzzy_cross_entropy(indptr, indices, data, n, emb, dim, a, b, n_random, seed, stream()), out);
+  });
+}
+
+umapgpu_status umapgpu_manifold_fuzzy_cross_entropy(const umapgpu_manifold* m, const float* emb, uint64_t dim,
+                                                    uint64_t n_random, uint64_t seed, umapgpu_ce_terms* out) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !out) throw Error(Err::InvalidArgument, "null handle");
+    const DeviceCsr& g = m->g.graph;
+    ce_out(fuzzy_cross_entropy(g.indptr.as<uint64_t>(), g.indices.as<uint32_t>(), g.data.as<float>(), g.n, emb, dim,
+                               m->a, m->b, n_random, seed, stream()),
+           out);
+  });
+}
+
+umapgpu_status umapgpu_optimizer_fuzzy_cross_entropy(const umapgpu_optimizer* o, uint64_t n_random, uint64_t seed,
+                                                     umapgpu_ce_terms* out) {
+  return guarded([&] {
+    ensure_device();
+    if (!o || !out) throw Error(Err::InvalidArgument, "null handle");
+    const DeviceCsr& g = o->manifold->g.graph;
+    ce_out(fuzzy_cross_entropy(g.indptr.as<uint64_t>(), g.indices.as<uint32_t>(), g.data.as<float>(), g.n,
+                               o->st.head_emb.as<float>(), o->st.dim, o->manifold->a, o->manifold->b, n_random, seed,
+                               stream()),
+           out);
+  });
+}
+
+umapgpu_status umapgpu_knn_query_device(const float* data, uint64_t n, uint64_t dim, const uint32_t* queries,
+                                        uint64_t nq, uint64_t k, uint32_t* out) {
+  return guarded([&] {
+    ensure_device();
+    knn_query(data, n, dim, queries, nq, k, out, stream());
+  });
+}
+
+umapgpu_status umapgpu_trustworthiness_penalty_device(const float* data, uint64_t n, uint64_t dim,
+                                                      const uint32_t* queries, uint64_t nq, const uint32_t* nbrs,
+                                                      uint64_t k, double* penalty) {
+  return guarded([&] {
+    ensure_device();
+    if (!penalty) throw Error(Err::InvalidArgument, "null output");
+    *penalty = trustworthiness_penalty(data, n, dim, queries, nq, nbrs, k, stream());
   });
 }
 

@@ -1,4 +1,5 @@
-// Exact brute-force kNN producer (benchmark and test inputs).
+// Exact brute-force kNN producer (benchmark and test inputs) and the
+// query form used by the evaluators.
 #pragma once
 #include "common.hpp"
 
@@ -7,5 +8,10 @@ namespace umapgpu {
 // X: n x dim row-major device array; outputs n x k (self at slot 0).
 void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32_t* out_idx, float* out_dist,
                     hipStream_t s);
+
+// k nearest other rows of X for each of the nq rows listed in `queries`
+// (device u32), self excluded, ties by lower index; out_idx is nq x k.
+void knn_query(const float* X, uint64_t n, uint64_t dim, const uint32_t* queries, uint64_t nq, uint64_t k,
+               uint32_t* out_idx, hipStream_t s);
 
 }  // namespace umapgpu

@@ -58,6 +58,14 @@ class _Config(C.Structure):
 
 DISTANCE_FN = C.CFUNCTYPE(f32, C.POINTER(f32), C.POINTER(f32), u64, C.POINTER(f32), P)
 
+
+class _CeTerms(C.Structure):
+    _fields_ = [("total", f64), ("attract", f64), ("repulse_sum", f64), ("edges", u64), ("pairs", u64)]
+
+    def as_dict(self) -> dict:
+        return {"total": self.total, "attract": self.attract, "repulse_sum": self.repulse_sum,
+                "edges": self.edges, "pairs": self.pairs}
+
 _lib = None
 
 
@@ -120,6 +128,12 @@ def lib():
             "umapgpu_memcpy_dtoh": ([P, P, u64], i32),
             "umapgpu_synchronize": ([], i32),
             "umapgpu_knn_bruteforce_device": ([P, u64, u64, u64, P, P], i32),
+            "umapgpu_fuzzy_cross_entropy_device": ([P, P, P, u64, P, u64, f32, f32, u64, u64,
+                                                    C.POINTER(_CeTerms)], i32),
+            "umapgpu_manifold_fuzzy_cross_entropy": ([P, P, u64, u64, u64, C.POINTER(_CeTerms)], i32),
+            "umapgpu_optimizer_fuzzy_cross_entropy": ([P, u64, u64, C.POINTER(_CeTerms)], i32),
+            "umapgpu_knn_query_device": ([P, u64, u64, P, u64, u64, P], i32),
+            "umapgpu_trustworthiness_penalty_device": ([P, u64, u64, P, u64, P, u64, C.POINTER(f64)], i32),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
             "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
@@ -357,6 +371,13 @@ class LearnedManifold(_Handle):
         _, _, a, b = self._info()
         return a, b
 
+    def fuzzy_cross_entropy(self, embedding: "DeviceArray", n_random: int = 200_000, seed: int = 3) -> dict:
+        """Fuzzy cross-entropy of a device embedding against this graph (evaluator)."""
+        t = _CeTerms()
+        _check(lib().umapgpu_manifold_fuzzy_cross_entropy(self._handle(), P(embedding.ptr), embedding.shape[1],
+                                                          n_random, seed, C.byref(t)))
+        return t.as_dict()
+
     def graph(self) -> SparseMat:
         nv, nnz, _, _ = self._info()
         ip = np.zeros(nv + 1, np.uint64)
@@ -535,6 +556,12 @@ class Optimizer(_Handle):
         _check(lib().umapgpu_optimizer_stats(self._handle(), C.byref(p), C.byref(n), C.byref(ms)))
         return p.value, n.value, ms.value
 
+    def fuzzy_cross_entropy(self, n_random: int = 200_000, seed: int = 3) -> dict:
+        """Fuzzy cross-entropy of the current embedding on the device (evaluator)."""
+        t = _CeTerms()
+        _check(lib().umapgpu_optimizer_fuzzy_cross_entropy(self._handle(), n_random, seed, C.byref(t)))
+        return t.as_dict()
+
     def export_params(self) -> dict:
         cur, tot, g, ia, nsr, mt, seed = u64(), u64(), f32(), f32(), f64(), i32(), u64()
         _check(lib().umapgpu_optimizer_export_params(self._handle(), C.byref(cur), C.byref(tot), C.byref(g),
@@ -708,6 +735,24 @@ def knn_bruteforce_device(data: DeviceArray, k: int):
     dst = DeviceArray((n, k), np.float32)
     _check(lib().umapgpu_knn_bruteforce_device(P(data.ptr), n, dim, k, P(idx.ptr), P(dst.ptr)))
     return idx, dst
+
+
+def knn_query_device(data: DeviceArray, queries: DeviceArray, k: int) -> DeviceArray:
+    """k nearest other rows of `data` for each listed query row (evaluators)."""
+    n, dim = data.shape
+    nq = queries.shape[0]
+    out = DeviceArray((nq, k), np.uint32)
+    _check(lib().umapgpu_knn_query_device(P(data.ptr), n, dim, P(queries.ptr), nq, k, P(out.ptr)))
+    return out
+
+
+def trustworthiness_penalty_device(data: DeviceArray, queries: DeviceArray, nbrs: DeviceArray) -> float:
+    n, dim = data.shape
+    nq, k = nbrs.shape
+    out = f64()
+    _check(lib().umapgpu_trustworthiness_penalty_device(P(data.ptr), n, dim, P(queries.ptr), nq, P(nbrs.ptr), k,
+                                                        C.byref(out)))
+    return out.value
 
 
 def learn_manifold_device(config: UmapConfig, knn_idx: DeviceArray, knn_dst: DeviceArray,

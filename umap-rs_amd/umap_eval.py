@@ -2,8 +2,10 @@
 
 * fuzzy_cross_entropy: the UMAP objective on the *unfiltered* symmetric graph:
       sum_{(i,j) in graph} w log(w/v) + (1-w) log((1-w)/(1-v))
-    + (n^2 - nnz)/M * sum_{M random non-edge pairs} -log(1-v)
-  with v = 1/(1 + a*|y_i - y_j|^(2b)) clamped to [1e-12, 1-1e-12].
+    + (n^2 - nnz)/M * sum_{M random pairs} -log(1-v)
+  with v = 1/(1 + a*|y_i - y_j|^(2b)) clamped to [1e-12, 1-1e-12].  Pair p is
+  (index(2p), index(2p+1)) of the SplitMix64 counter stream keyed by the seed,
+  the same stream the device evaluator draws, so both compute one quantity.
 * knn_preservation: mean over a fixed subsample of |kNN_high(i) & kNN_2d(i)|/k,
   kNN_high from the given kNN graph (self excluded), kNN_2d exact over all n.
 * trustworthiness: the rank-based formula (Venna & Kaski 2001), as in
@@ -12,10 +14,36 @@
 
 All use float64 on the host and fixed seeds, so repeated calls agree.  Rows
 whose embedding is NaN (fit marks disconnected vertices that way) are skipped.
+The *_gpu forms compute the same measures on the device through libumapgpu.
 """
 from __future__ import annotations
 
 import numpy as np
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_G2 = np.uint64(0xD1B54A32D192ED03)
+
+
+def _mix(z):
+    """SplitMix64's output function on uint64 (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        z = np.asarray(z, np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def counter_pairs(n: int, m: int, seed: int):
+    """The evaluator's random pair stream: pair p = (index(2p), index(2p+1)),
+    index(c) = floor(mix(key + (c+1)*G) * n / 2^64), key = mix(seed + G2)."""
+    with np.errstate(over="ignore"):
+        key = _mix(np.uint64(seed) + _G2)
+        c = np.arange(2 * m, dtype=np.uint64)
+        r = _mix(key + (c + np.uint64(1)) * _G)
+        hi, lo = r >> np.uint64(32), r & np.uint64(0xFFFFFFFF)
+        t = (lo * np.uint64(n)) >> np.uint64(32)
+        idx = ((hi * np.uint64(n) + t) >> np.uint64(32)).astype(np.int64)
+    return idx[0::2], idx[1::2]
 
 
 def _v(y, i, j, a, b):
@@ -35,9 +63,7 @@ def fuzzy_cross_entropy(indptr, indices, data, embedding, a, b, n_random=200_000
     rows, cols, w = rows[ok], cols[ok], w[ok]
     v = _v(y, rows, cols, float(a), float(b))
     attract = float(np.sum(w * np.log(w / v) + (1.0 - w) * np.log((1.0 - w) / (1.0 - v))))
-    rng = np.random.default_rng(seed)
-    i = rng.integers(0, n, n_random)
-    j = rng.integers(0, n, n_random)
+    i, j = counter_pairs(n, n_random, seed)
     keep = (i != j) & np.isfinite(y[i]).all(1) & np.isfinite(y[j]).all(1)
     i, j = i[keep], j[keep]
     rep = -np.log(1.0 - _v(y, i, j, float(a), float(b)))
@@ -104,3 +130,46 @@ def trustworthiness(x, embedding, k=15, m=10_000, seed=3):
             penalty += float(np.sum(np.maximum(rk - k, 0)))
     mm = len(q)
     return 1.0 - 2.0 / (mm * k * (2.0 * n - 3.0 * k - 1.0)) * penalty
+
+
+# ---------------------------------------------------------------------------
+# device forms (libumapgpu)
+
+def fuzzy_cross_entropy_gpu(manifold, embedding, n_random=200_000, seed=3) -> float:
+    """`embedding` is a DeviceArray (n x d) or an Optimizer (its current embedding)."""
+    if hasattr(embedding, "fuzzy_cross_entropy"):
+        return embedding.fuzzy_cross_entropy(n_random, seed)["total"]
+    return manifold.fuzzy_cross_entropy(embedding, n_random, seed)["total"]
+
+
+def knn_preservation_gpu(knn_indices, embedding, k=15, m=10_000, seed=3) -> float:
+    """knn_indices: host n x K array (self at slot 0); embedding: DeviceArray."""
+    import umap_amd as U
+    idx = np.asarray(knn_indices)
+    n = idx.shape[0]
+    q = _subsample(n, m, seed).astype(np.uint32)
+    dq = U.DeviceArray.from_host(q)
+    low = U.knn_query_device(embedding, dq, k).to_host()
+    dq.free()
+    y = embedding.to_host()
+    fin = np.isfinite(y[q]).all(1)
+    tot = 0.0
+    for r in np.nonzero(fin)[0]:
+        i = int(q[r])
+        high = [c for c in idx[i] if c != i][:k]
+        tot += len(set(high) & set(low[r].tolist())) / float(k)
+    return tot / max(int(fin.sum()), 1)
+
+
+def trustworthiness_gpu(x, embedding, k=15, m=10_000, seed=3) -> float:
+    """x, embedding: DeviceArrays (n x D, n x d)."""
+    import umap_amd as U
+    n = x.shape[0]
+    q = _subsample(n, m, seed).astype(np.uint32)
+    dq = U.DeviceArray.from_host(q)
+    low = U.knn_query_device(embedding, dq, k)
+    pen = U.trustworthiness_penalty_device(x, dq, low)
+    dq.free()
+    low.free()
+    mm = len(q)
+    return 1.0 - 2.0 / (mm * k * (2.0 * n - 3.0 * k - 1.0)) * pen
