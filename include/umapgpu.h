@@ -257,6 +257,56 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);
+/* ---- learned initialisation (no reference counterpart: the reference's fits
+ * start from the caller's init, embedding.rs:274-298).  A D -> 256 -> 256 ->
+ * out_dim ReLU MLP trained on the embedding a plain fit produced for a
+ * subsample, then used to predict the initial layout of every point.
+ * Forward and backward run on bf16 MFMA with f32 accumulation; master
+ * weights and the Adam moments are f32.  Inputs are standardised per
+ * feature and targets per dimension (mean) and jointly (scale), both with
+ * statistics fixed by umapgpu_mlp_fit_normalization or set explicitly.
+ * Loss: mean over the batch and output dimensions of the squared error in
+ * standardised target units. ---- */
+typedef struct umapgpu_mlp umapgpu_mlp;
+typedef struct umapgpu_mlp_config {
+  uint64_t in_dim;     /* D, 1..4096 */
+  uint64_t out_dim;    /* embedding dimension, 1..8 */
+  uint64_t batch;      /* rows per Adam step, a multiple of 64 */
+  float learning_rate; /* Adam */
+  float beta1, beta2, eps;
+  uint64_t seed;       /* weight init and per-epoch shuffles */
+} umapgpu_mlp_config;
+/* in_dim 0 (must be set), out_dim 2, batch 4096, lr 1e-3, betas (0.9, 0.999),
+ * eps 1e-8, seed 0. */
+UMAPGPU_API void umapgpu_mlp_config_default(umapgpu_mlp_config* cfg);
+UMAPGPU_API umapgpu_status umapgpu_mlp_create(const umapgpu_mlp_config* cfg, umapgpu_mlp** out);
+UMAPGPU_API void umapgpu_mlp_destroy(umapgpu_mlp* m);
+/* Flat f32 parameters: W1 [256 x D], b1 [256], W2 [256 x 256], b2 [256],
+ * W3 [out_dim x 256], b3 [out_dim]; rows are output units. */
+UMAPGPU_API uint64_t umapgpu_mlp_param_count(const umapgpu_mlp* m);
+UMAPGPU_API umapgpu_status umapgpu_mlp_get_params(const umapgpu_mlp* m, float* params_host);
+/* Also clears the Adam moments and step count. */
+UMAPGPU_API umapgpu_status umapgpu_mlp_set_params(umapgpu_mlp* m, const float* params_host);
+/* Statistics of n rows of x (n x D) and y (n x out_dim), device arrays. */
+UMAPGPU_API umapgpu_status umapgpu_mlp_fit_normalization(umapgpu_mlp* m, const float* x_device,
+                                                         const float* y_device, uint64_t n);
+UMAPGPU_API umapgpu_status umapgpu_mlp_set_normalization(umapgpu_mlp* m, const float* x_mean, const float* x_inv_std,
+                                                         const float* y_mean, float y_scale);
+UMAPGPU_API umapgpu_status umapgpu_mlp_get_normalization(const umapgpu_mlp* m, float* x_mean, float* x_inv_std,
+                                                         float* y_mean, float* y_scale);
+/* Loss and gradient (to the host) of the batch made of rows 0..n-1 (n <=
+ * batch); the parameters do not change. */
+UMAPGPU_API umapgpu_status umapgpu_mlp_gradients(umapgpu_mlp* m, const float* x_device, const float* y_device,
+                                                 uint64_t n, float* grad_host, double* loss);
+/* `epochs` shuffled passes over rows 0..n-1 with one Adam step per batch;
+ * *last_epoch_loss is the mean loss over the last pass. */
+UMAPGPU_API umapgpu_status umapgpu_mlp_train(umapgpu_mlp* m, const float* x_device, const float* y_device,
+                                             uint64_t n, uint64_t epochs, double* last_epoch_loss);
+/* y (n x out_dim, device) = the network at x (n x D, device), in target units. */
+UMAPGPU_API umapgpu_status umapgpu_mlp_predict(const umapgpu_mlp* m, const float* x_device, uint64_t n,
+                                               float* y_device);
+UMAPGPU_API umapgpu_status umapgpu_mlp_stats(const umapgpu_mlp* m, uint64_t* adam_steps, double* train_ms);
+
 /* ---- evaluators (no reference counterpart: the reference defines no quality
  * measure).  Used to judge embeddings and to time wall-to-target-loss. ---- */
 typedef struct umapgpu_ce_terms {

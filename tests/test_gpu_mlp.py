@@ -1,0 +1,137 @@
+"""Device MLP (bf16 MFMA forward/backward, Adam) against the CPU oracle.
+
+Tolerances: against the oracle with the device's bf16 rounding points only
+the f32 summation order differs, so gradients agree to 1e-2 relative (L2 per
+parameter block, typically ~1e-3); against the pure FP64 oracle the bf16
+rounding itself enters, 1e-1 (0.06 seen for W1 at D=300 and 64 rows).
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks(D, O):
+    lay = M.layout(D, O)
+    del lay["total"]
+    return {k: slice(o, o + int(np.prod(s))) for k, (o, s) in lay.items()}
+
+
+def _setup(U, D, O, n, batch, seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.normal(0, 1.5, (n, D)) + rng.normal(0, 3, D)).astype(np.float32)
+    t = (np.tanh(x[:, :O] * 0.7) * 4 + 0.3 * x[:, -O:] + 2.0).astype(np.float32)
+    mlp = U.Mlp(U.MlpConfig(in_dim=D, out_dim=O, batch=batch, seed=seed))
+    p = mlp.get_params()
+    b = _blocks(D, O)
+    for k in ("b1", "b2", "b3"):  # nonzero biases exercise the bias paths
+        p[b[k]] = rng.normal(0, 0.05, p[b[k]].size).astype(np.float32)
+    mlp.set_params(p)
+    xm, xi = x.mean(0), (1.0 / x.std(0)).astype(np.float32)
+    ym, ys = t.mean(0), float(t.std())
+    mlp.set_normalization(xm, xi, ym, ys)
+    return mlp, x, t, p, (xm, xi, ym, ys)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("D,O,n,batch", [(40, 2, 200, 256), (128, 2, 1024, 1024), (64, 3, 777, 1024),
+                                         (300, 2, 64, 64)])
+def test_gpu_mlp_gradients_match_oracle(gpu, D, O, n, batch):
+    U = gpu
+    mlp, x, t, p, (xm, xi, ym, ys) = _setup(U, D, O, n, batch, seed=D + n)
+    dx, dt = U.DeviceArray.from_host(x), U.DeviceArray.from_host(t)
+    loss, g = mlp.gradients(dx, dt, n)
+    l16, g16 = M.forward_backward(p, x, t, xm, xi, ym, ys, bf16_points=True)
+    l64, g64 = M.forward_backward(p, x, t, xm, xi, ym, ys)
+    assert loss == pytest.approx(l16, rel=2e-3)
+    assert loss == pytest.approx(l64, rel=3e-2)
+    for k, sl in _blocks(D, O).items():
+        assert _rel(g[sl], g16[sl]) < 1e-2, (k, _rel(g[sl], g16[sl]))
+        assert _rel(g[sl], g64[sl]) < 1e-1, (k, _rel(g[sl], g64[sl]))
+    assert np.array_equal(mlp.get_params(), p)  # a gradient call changes nothing
+    dx.free()
+    dt.free()
+
+
+def test_gpu_mlp_predict_matches_forward(gpu):
+    U = gpu
+    D, O, n = 96, 2, 5000
+    mlp, x, t, p, (xm, xi, ym, ys) = _setup(U, D, O, n, 1024, seed=5)
+    dx = U.DeviceArray.from_host(x)
+    y = mlp.predict(dx).to_host()
+    y16, _ = M.forward(p, x, xm, xi, O, bf16_points=True)
+    y64, _ = M.forward(p, x, xm, xi, O)
+    y16 = y16 * ys + ym
+    y64 = y64 * ys + ym
+    scale = np.abs(y64).max()
+    assert np.abs(y - y16).max() <= 2e-3 * scale
+    assert np.abs(y - y64).max() <= 3e-2 * scale
+    dx.free()
+
+
+def test_gpu_mlp_adam_step(gpu):
+    """One training epoch whose single batch is every row equals one Adam step
+    on the gradient of those rows."""
+    U = gpu
+    D, O, n = 48, 2, 512
+    mlp, x, t, p, _ = _setup(U, D, O, n, 512, seed=7)
+    dx, dt = U.DeviceArray.from_host(x), U.DeviceArray.from_host(t)
+    _, g = mlp.gradients(dx, dt, n)
+    mlp.train(dx, dt, epochs=1)
+    got = mlp.get_params()
+    want, _, _ = M.adam(p, np.zeros_like(p), np.zeros_like(p), g, 1)
+    err = np.abs(got - want)
+    # rows are summed in another order inside the shuffled batch; Adam's first
+    # step is ~lr*sign(g), so only entries with g ~ 0 may differ
+    assert np.quantile(err, 0.99) <= 1e-6
+    assert err.max() <= 2.0e-3
+    assert mlp.stats()[0] == 1
+
+
+def test_gpu_mlp_training_fits_a_smooth_map(gpu):
+    U = gpu
+    D, O, n = 32, 2, 20000
+    rng = np.random.default_rng(11)
+    x = rng.normal(0, 1, (n, D)).astype(np.float32)
+    proj = rng.normal(0, 1, (D, O)).astype(np.float32) / np.sqrt(D)
+    t = (np.sin(x @ proj * 1.5) * 5.0).astype(np.float32)
+    mlp = U.Mlp(U.MlpConfig(in_dim=D, out_dim=O, batch=1024, seed=1))
+    dx, dt = U.DeviceArray.from_host(x), U.DeviceArray.from_host(t)
+    mlp.fit_normalization(dx, dt)
+    first = mlp.train(dx, dt, epochs=1)
+    last = mlp.train(dx, dt, epochs=60)
+    assert last < 0.1 * first, (first, last)
+    y = mlp.predict(dx).to_host()
+    assert np.sqrt(np.mean((y - t) ** 2)) < 0.35 * t.std()
+    steps, ms = mlp.stats()
+    assert steps == 61 * 20 and ms > 0
+
+
+def test_gpu_mlp_normalization_stats(gpu):
+    U = gpu
+    rng = np.random.default_rng(3)
+    x = (rng.normal(0, 2, (3000, 20)) + 5).astype(np.float32)
+    x[:, 7] = 1.25  # constant feature: inverse scale falls back to 1
+    y = rng.normal(0, 3, (3000, 2)).astype(np.float32)
+    mlp = U.Mlp(U.MlpConfig(in_dim=20, out_dim=2, batch=256))
+    mlp.fit_normalization(U.DeviceArray.from_host(x), U.DeviceArray.from_host(y))
+    xm, xi, ym, ys = mlp.get_normalization()
+    np.testing.assert_allclose(xm, x.astype(np.float64).mean(0), rtol=1e-6)
+    sd = x.astype(np.float64).std(0)
+    np.testing.assert_allclose(xi[sd > 0], 1.0 / sd[sd > 0], rtol=1e-5)
+    assert xi[7] == 1.0
+    np.testing.assert_allclose(ym, y.astype(np.float64).mean(0), rtol=1e-6, atol=1e-6)
+    assert ys == pytest.approx(np.sqrt(np.mean(y.astype(np.float64).var(0))), rel=1e-6)
+
+
+def test_mlp_config_errors(gpu):
+    U = gpu
+    with pytest.raises(U.UmapPanic, match="multiple of 64"):
+        U.Mlp(U.MlpConfig(in_dim=8, batch=100))
+    with pytest.raises(U.UmapPanic, match="output dimension"):
+        U.Mlp(U.MlpConfig(in_dim=8, out_dim=9))

@@ -13,6 +13,7 @@
 #include "evaluate.hpp"
 #include "graph.hpp"
 #include "knn.hpp"
+#include "mlp.hpp"
 #include "optimize.hpp"
 #include "primitives.hpp"
 
@@ -222,6 +223,10 @@ struct umapgpu_manifold {
 struct umapgpu_optimizer {
   OptimizerState st;
   std::unique_ptr<umapgpu_manifold> manifold;
+};
+
+struct umapgpu_mlp {
+  MlpState st;
 };
 
 struct umapgpu_fitted {
@@ -867,6 +872,119 @@ umapgpu_status umapgpu_knn_bruteforce_device(const float* data, uint64_t n, uint
   return guarded([&] {
     ensure_device();
     knn_bruteforce(data, n, dim, k, idx, dists, stream());
+  });
+}
+
+void umapgpu_mlp_config_default(umapgpu_mlp_config* c) {
+  if (!c) return;
+  MlpConfig d;
+  c->in_dim = 0;
+  c->out_dim = d.out_dim;
+  c->batch = d.batch;
+  c->learning_rate = d.lr;
+  c->beta1 = d.beta1;
+  c->beta2 = d.beta2;
+  c->eps = d.eps;
+  c->seed = d.seed;
+}
+
+umapgpu_status umapgpu_mlp_create(const umapgpu_mlp_config* cfg, umapgpu_mlp** out) {
+  return guarded([&] {
+    ensure_device();
+    if (!cfg || !out) throw Error(Err::InvalidArgument, "null argument");
+    MlpConfig c;
+    c.in_dim = cfg->in_dim;
+    c.out_dim = cfg->out_dim;
+    c.batch = cfg->batch;
+    c.lr = cfg->learning_rate;
+    c.beta1 = cfg->beta1;
+    c.beta2 = cfg->beta2;
+    c.eps = cfg->eps;
+    c.seed = cfg->seed;
+    auto m = std::make_unique<umapgpu_mlp>();
+    mlp_create(c, &m->st, stream());
+    *out = m.release();
+  });
+}
+
+void umapgpu_mlp_destroy(umapgpu_mlp* m) { delete m; }
+
+uint64_t umapgpu_mlp_param_count(const umapgpu_mlp* m) { return m ? m->st.lay.total : 0; }
+
+umapgpu_status umapgpu_mlp_get_params(const umapgpu_mlp* m, float* host) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !host) throw Error(Err::InvalidArgument, "null argument");
+    mlp_get_params(m->st, host, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_set_params(umapgpu_mlp* m, const float* host) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !host) throw Error(Err::InvalidArgument, "null argument");
+    mlp_set_params(&m->st, host, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_fit_normalization(umapgpu_mlp* m, const float* x, const float* y, uint64_t n) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !x || !y) throw Error(Err::InvalidArgument, "null argument");
+    mlp_fit_normalization(&m->st, x, y, n, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_set_normalization(umapgpu_mlp* m, const float* xm, const float* xi, const float* ym,
+                                             float ys) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !xm || !xi || !ym) throw Error(Err::InvalidArgument, "null argument");
+    mlp_set_normalization(&m->st, xm, xi, ym, ys, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_get_normalization(const umapgpu_mlp* m, float* xm, float* xi, float* ym, float* ys) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !xm || !xi || !ym || !ys) throw Error(Err::InvalidArgument, "null argument");
+    mlp_get_normalization(m->st, xm, xi, ym, ys, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_gradients(umapgpu_mlp* m, const float* x, const float* y, uint64_t n, float* grad,
+                                     double* loss) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !x || !y || !grad) throw Error(Err::InvalidArgument, "null argument");
+    const double l = mlp_gradients(&m->st, x, y, n, grad, stream());
+    if (loss) *loss = l;
+  });
+}
+
+umapgpu_status umapgpu_mlp_train(umapgpu_mlp* m, const float* x, const float* y, uint64_t n, uint64_t epochs,
+                                 double* last) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || !x || !y) throw Error(Err::InvalidArgument, "null argument");
+    const double l = mlp_train(&m->st, x, y, n, epochs, stream());
+    if (last) *last = l;
+  });
+}
+
+umapgpu_status umapgpu_mlp_predict(const umapgpu_mlp* m, const float* x, uint64_t n, float* y) {
+  return guarded([&] {
+    ensure_device();
+    if (!m || (n && (!x || !y))) throw Error(Err::InvalidArgument, "null argument");
+    mlp_predict(m->st, x, n, y, stream());
+  });
+}
+
+umapgpu_status umapgpu_mlp_stats(const umapgpu_mlp* m, uint64_t* steps, double* ms) {
+  return guarded([&] {
+    if (!m) throw Error(Err::InvalidArgument, "null argument");
+    if (steps) *steps = m->st.step;
+    if (ms) *ms = m->st.train_ms;
   });
 }
 

@@ -1,0 +1,770 @@
+// Learned-initialisation MLP on bf16 MFMA (gfx950 v_mfma_f32_32x32x16_bf16).
+//
+// One Adam step over a batch of B rows is three launches:
+//   1. mlp_rows_kernel<TRAIN>: a block owns 64 rows.  Layer 1 streams the
+//      normalised input through LDS in 64-column chunks; layers 1 and 2 are
+//      64x256 MFMA tiles (wave w owns columns 64w..64w+63) whose activations
+//      stay in LDS as bf16; layer 3 (256 -> O) is a short f32 dot product.
+//      The same block then runs the data backward: dY from the squared error,
+//      dH2 = (dY W3) * [H2 > 0], dH1 = (dH2 W2) * [H1 > 0] on MFMA.  Inputs,
+//      activations and their gradients leave the block feature-major
+//      ([feature][row], bf16) so that the weight gradients read them with
+//      unit stride.
+//   2. mlp_wgrad_kernel: dW = dH^T H as 64x64 MFMA tiles, split over 512-row
+//      slices of the batch into per-slice partials (no atomics, so a step is
+//      deterministic); one extra block per slice does the biases and W3.
+//   3. mlp_adam_kernel: sums the slice partials in order, applies Adam to the
+//      f32 master weights and refreshes the bf16 copies (W1, W2 and W2^T, the
+//      last feeding the dH1 product).
+// Prediction is mlp_rows_kernel<false> over all rows.
+//
+// MFMA operand layout (32x32x16): lane l holds A[l%32][8(l/32)..+8] and
+// B[8(l/32)..+8][l%32]; accumulator element v of lane l is
+// C[8(v/4) + 4(l/32) + v%4][l%32].
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mlp.hpp"
+
+namespace umapgpu {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kH = kMlpHidden;
+constexpr int kR = 64;           // rows per block
+constexpr int kKC = 64;          // layer-1 input columns per LDS chunk
+constexpr int kLdX = kKC + 8;    // LDS row strides (bf16), padded against bank conflicts
+constexpr int kLdH = kH + 8;
+constexpr int kKS = 512;         // batch rows per weight-gradient slice
+constexpr int kThreads = 256;
+
+// f32 -> bf16, round to nearest even (NaN stays NaN).
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline bf16x8 ld8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const u32x4*)p); }
+
+__device__ inline f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+struct RowsArgs {
+  const float* X;          // [rows x D] f32
+  uint32_t D, Dp, O;
+  const uint32_t* ids;     // batch row ids (training) or null: row = tile row
+  uint32_t nrows;          // valid rows
+  const float* xmean;
+  const float* xinv;
+  const uint16_t* w1b;     // [256 x Dp]
+  const uint16_t* w2b;     // [256 x 256]
+  const uint16_t* w2tb;    // [256 x 256], W2 transposed
+  const float* params;     // f32 master (biases and W3 read here)
+  uint32_t off_b1, off_b2, off_w3, off_b3;
+  const float* ymean;      // [O]
+  // prediction
+  float* Y;                // [rows x O]
+  float yscale;
+  // training
+  const float* T;          // [rows x O] targets (original units)
+  float yinv;              // 1 / yscale
+  float dy_scale;          // 2 / (valid rows * O)
+  uint32_t ldb;            // batch capacity (row stride of the feature-major buffers)
+  uint16_t *xt, *h1t, *h2t, *dh1t, *dh2t;
+  float* dyt;              // [O x ldb]
+  float* loss_part;        // per block
+};
+
+// acc[2][2] += A(64 x K, LDS, row stride lda) * B^T where B rows are
+// columns n0..n0+63 of the layer (global, row stride ldw), k in [k0, k0+kn).
+__device__ inline void tile_mfma(f32x16 (&acc)[2][2], const uint16_t* a_lds, int lda, int a_k0,
+                                 const uint16_t* w, uint64_t ldw, int n0, int w_k0, int kn, int lane) {
+  const int r = lane & 31, h = (lane >> 5) * 8;
+  for (int kk = 0; kk < kn; kk += 16) {
+    bf16x8 af[2], bw[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) bw[nt] = ld8(w + (uint64_t)(n0 + nt * 32 + r) * ldw + w_k0 + kk + h);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) af[mt] = ld8(a_lds + (mt * 32 + r) * lda + a_k0 + kk + h);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma(af[mt], bw[nt], acc[mt][nt]);
+  }
+}
+
+// Bias + ReLU epilogue into an LDS bf16 tile, and (training) the same values
+// feature-major to global: the 4 rows v%4 = 0..3 of a lane are contiguous.
+template <bool TRAIN>
+__device__ inline void relu_out(const f32x16 (&acc)[2][2], const float* __restrict__ bias, uint16_t* lds,
+                                uint16_t* __restrict__ gt, uint32_t ldb, uint32_t m0, int n0, int lane) {
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int col = n0 + nt * 32 + (lane & 31);
+      const float bv = bias[col];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        uint16_t q[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float x = acc[mt][nt][4 * g + e] + bv;
+          q[e] = f2bf(x > 0.0f ? x : 0.0f);
+          lds[(mt * 32 + 8 * g + 4 * (lane >> 5) + e) * kLdH + col] = q[e];
+        }
+        if (TRAIN) {
+          u32x2 pk = {(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+          *(u32x2*)(gt + (uint64_t)col * ldb + m0 + mt * 32 + 8 * g + 4 * (lane >> 5)) = pk;
+        }
+      }
+    }
+}
+
+template <bool TRAIN>
+__global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
+  __shared__ __align__(16) uint16_t xs[kR * kLdX];
+  __shared__ __align__(16) uint16_t h1[kR * kLdH];
+  __shared__ __align__(16) uint16_t h2[kR * kLdH];
+  __shared__ __align__(16) uint16_t dh2[TRAIN ? kR * kLdH : 8];
+  __shared__ float w3[kMlpMaxOut * kH];
+  __shared__ float dys[kR * kMlpMaxOut];
+  __shared__ uint32_t rid[kR];
+  __shared__ float red[kThreads / 64];
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t m0 = blockIdx.x * kR;
+  const int n0 = wave * 64;
+  const uint32_t O = a.O;
+  if (t < kR) {
+    const uint32_t m = m0 + t;
+    rid[t] = m < a.nrows ? (a.ids ? a.ids[m] : m) : 0xffffffffu;
+  }
+  for (uint32_t e = t; e < O * kH; e += kThreads) w3[e] = a.params[a.off_w3 + e];
+  __syncthreads();
+
+  // ---- layer 1
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+  for (uint32_t k0 = 0; k0 < a.Dp; k0 += kKC) {
+    {  // stage: thread owns column c of 16 rows
+      const uint32_t c = k0 + (t & 63);
+      const int r0 = (t >> 6) * 16;
+      const bool cin = c < a.D;
+      const float mu = cin ? a.xmean[c] : 0.0f, iv = cin ? a.xinv[c] : 0.0f;
+      uint16_t q[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const uint32_t row = rid[r0 + e];
+        float x = 0.0f;
+        if (cin && row != 0xffffffffu) x = (a.X[(uint64_t)row * a.D + c] - mu) * iv;
+        q[e] = f2bf(x);
+        xs[(r0 + e) * kLdX + (t & 63)] = q[e];
+      }
+      if (TRAIN) {
+        u32x4 lo, hi;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          lo[e] = (uint32_t)q[2 * e] | ((uint32_t)q[2 * e + 1] << 16);
+          hi[e] = (uint32_t)q[8 + 2 * e] | ((uint32_t)q[8 + 2 * e + 1] << 16);
+        }
+        uint16_t* dst = a.xt + (uint64_t)c * a.ldb + m0 + r0;
+        *(u32x4*)dst = lo;
+        *(u32x4*)(dst + 8) = hi;
+      }
+    }
+    __syncthreads();
+    tile_mfma(acc, xs, kLdX, 0, a.w1b, a.Dp, n0, (int)k0, kKC, lane);
+    __syncthreads();
+  }
+  relu_out<TRAIN>(acc, a.params + a.off_b1, h1, a.h1t, a.ldb, m0, n0, lane);
+  __syncthreads();
+
+  // ---- layer 2
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+  tile_mfma(acc, h1, kLdH, 0, a.w2b, kH, n0, 0, kH, lane);
+  relu_out<TRAIN>(acc, a.params + a.off_b2, h2, a.h2t, a.ldb, m0, n0, lane);
+  __syncthreads();
+
+  // ---- layer 3: four lanes per row, each a quarter of the 256 inputs
+  const int r = t >> 2, qtr = t & 3;
+  float y[kMlpMaxOut];
+#pragma unroll
+  for (int o = 0; o < kMlpMaxOut; ++o) y[o] = 0.0f;
+  for (int j = qtr * 64; j < qtr * 64 + 64; ++j) {
+    const float hv = bf2f(h2[r * kLdH + j]);
+#pragma unroll
+    for (int o = 0; o < kMlpMaxOut; ++o)
+      if (o < (int)O) y[o] = y[o] + hv * w3[o * kH + j];
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpMaxOut; ++o) {
+    y[o] = y[o] + __shfl_xor(y[o], 1, 64);
+    y[o] = y[o] + __shfl_xor(y[o], 2, 64);
+  }
+  const uint32_t row = rid[r];
+  if (!TRAIN) {
+    if (qtr == 0 && row != 0xffffffffu)
+      for (uint32_t o = 0; o < O; ++o)
+        a.Y[(uint64_t)row * O + o] = (y[o] + a.params[a.off_b3 + o]) * a.yscale + a.ymean[o];
+    return;
+  }
+
+  // ---- loss and dY
+  float sq = 0.0f;
+  if (qtr == 0) {
+    for (uint32_t o = 0; o < O; ++o) {
+      float d = 0.0f;
+      if (row != 0xffffffffu) {
+        const float tn = (a.T[(uint64_t)row * O + o] - a.ymean[o]) * a.yinv;
+        d = (y[o] + a.params[a.off_b3 + o]) - tn;
+      }
+      sq = sq + d * d;
+      const float g = a.dy_scale * d;
+      dys[r * kMlpMaxOut + o] = g;
+      a.dyt[(uint64_t)o * a.ldb + m0 + r] = g;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+  if (lane == 0) red[wave] = sq;
+  __syncthreads();
+  if (t == 0) a.loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+
+  // ---- dH2 = (dY W3) * [H2 > 0]: thread owns column j, 8 rows per store
+  {
+    const int j = t;
+    for (int r0 = 0; r0 < kR; r0 += 8) {
+      uint16_t q[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int rr = r0 + e;
+        float g = 0.0f;
+        for (uint32_t o = 0; o < O; ++o) g = g + dys[rr * kMlpMaxOut + o] * w3[o * kH + j];
+        q[e] = h2[rr * kLdH + j] != 0 ? f2bf(g) : (uint16_t)0;
+        dh2[rr * kLdH + j] = q[e];
+      }
+      u32x4 pk;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pk[e] = (uint32_t)q[2 * e] | ((uint32_t)q[2 * e + 1] << 16);
+      *(u32x4*)(a.dh2t + (uint64_t)j * a.ldb + m0 + r0) = pk;
+    }
+  }
+  __syncthreads();
+
+  // ---- dH1 = (dH2 W2) * [H1 > 0]: B operand rows are columns of W2 (W2^T rows)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) acc[i][jj] = f32x16{};
+  tile_mfma(acc, dh2, kLdH, 0, a.w2tb, kH, n0, 0, kH, lane);
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int col = n0 + nt * 32 + (lane & 31);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        uint16_t q[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int rr = mt * 32 + 8 * g + 4 * (lane >> 5) + e;
+          q[e] = h1[rr * kLdH + col] != 0 ? f2bf(acc[mt][nt][4 * g + e]) : (uint16_t)0;
+        }
+        u32x2 pk = {(uint32_t)q[0] | ((uint32_t)q[1] << 16), (uint32_t)q[2] | ((uint32_t)q[3] << 16)};
+        *(u32x2*)(a.dh1t + (uint64_t)col * a.ldb + m0 + mt * 32 + 8 * g + 4 * (lane >> 5)) = pk;
+      }
+    }
+}
+
+struct WgradArgs {
+  const uint16_t *xt, *h1t, *h2t, *dh1t, *dh2t;
+  const float* dyt;
+  uint32_t ldb, kc;        // row stride; batch columns in use (multiple of 64)
+  uint32_t D, Dp, O;
+  uint32_t w1_tiles;       // 4 * Dp / 64
+  float* part;             // [slices x P]
+  uint64_t P;
+  uint32_t off_w1, off_b1, off_w2, off_b2, off_w3, off_b3;
+};
+
+// grid.x: 16 W2 tiles, then the W1 tiles, then one vector block; grid.y: slice.
+__global__ __launch_bounds__(kThreads) void mlp_wgrad_kernel(WgradArgs a) {
+  const uint32_t task = blockIdx.x, s = blockIdx.y;
+  const uint32_t kb = s * kKS, ke = min(kb + (uint32_t)kKS, a.kc);
+  float* out = a.part + (uint64_t)s * a.P;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (task < 16 + a.w1_tiles) {
+    const bool is_w2 = task < 16;
+    const uint32_t tt = is_w2 ? task : task - 16;
+    const uint32_t ntn = is_w2 ? 4 : a.Dp / 64;
+    const uint32_t o0 = (tt / ntn) * 64 + (wave >> 1) * 32, i0 = (tt % ntn) * 64 + (wave & 1) * 32;
+    const uint16_t* A = (is_w2 ? a.dh2t : a.dh1t) + (uint64_t)(o0 + (lane & 31)) * a.ldb + (lane >> 5) * 8;
+    const uint16_t* B = (is_w2 ? a.h1t : a.xt) + (uint64_t)(i0 + (lane & 31)) * a.ldb + (lane >> 5) * 8;
+    f32x16 acc0 = {}, acc1 = {};
+    uint32_t k = kb;
+    for (; k + 32 <= ke; k += 32) {
+      acc0 = mfma(ld8(A + k), ld8(B + k), acc0);
+      acc1 = mfma(ld8(A + k + 16), ld8(B + k + 16), acc1);
+    }
+    for (; k < ke; k += 16) acc0 = mfma(ld8(A + k), ld8(B + k), acc0);
+    const uint32_t i = i0 + (lane & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const uint32_t o = o0 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+      const float g = acc0[v] + acc1[v];
+      if (is_w2)
+        out[a.off_w2 + (uint64_t)o * kH + i] = g;
+      else if (i < a.D)
+        out[a.off_w1 + (uint64_t)o * a.D + i] = g;
+    }
+    return;
+  }
+  // biases and W3: thread owns hidden unit j
+  const uint32_t j = t;
+  float g1 = 0.0f, g2 = 0.0f, g3[kMlpMaxOut];
+#pragma unroll
+  for (int o = 0; o < kMlpMaxOut; ++o) g3[o] = 0.0f;
+  for (uint32_t m = kb; m < ke; m += 8) {
+    const u32x4 p1 = *(const u32x4*)(a.dh1t + (uint64_t)j * a.ldb + m);
+    const u32x4 p2 = *(const u32x4*)(a.dh2t + (uint64_t)j * a.ldb + m);
+    const u32x4 p3 = *(const u32x4*)(a.h2t + (uint64_t)j * a.ldb + m);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float d1 = bf2f((uint16_t)(p1[e / 2] >> (16 * (e & 1))));
+      const float d2 = bf2f((uint16_t)(p2[e / 2] >> (16 * (e & 1))));
+      const float hv = bf2f((uint16_t)(p3[e / 2] >> (16 * (e & 1))));
+      g1 = g1 + d1;
+      g2 = g2 + d2;
+#pragma unroll
+      for (int o = 0; o < kMlpMaxOut; ++o)
+        if (o < (int)a.O) g3[o] = g3[o] + a.dyt[(uint64_t)o * a.ldb + m + e] * hv;
+    }
+  }
+  out[a.off_b1 + j] = g1;
+  out[a.off_b2 + j] = g2;
+  for (uint32_t o = 0; o < a.O; ++o) out[a.off_w3 + (uint64_t)o * kH + j] = g3[o];
+  if (j < a.O) {
+    float g = 0.0f;
+    for (uint32_t m = kb; m < ke; ++m) g = g + a.dyt[(uint64_t)j * a.ldb + m];
+    out[a.off_b3 + j] = g;
+  }
+}
+
+struct AdamArgs {
+  float *w, *m, *v;
+  const float* part;
+  uint64_t P;
+  uint32_t slices;   // 0: no gradient (only refresh the bf16 copies)
+  int update;        // apply Adam
+  float* grad_out;   // optional: the summed gradient
+  float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+  uint16_t *w1b, *w2b, *w2tb;
+  uint32_t D, Dp;
+  uint32_t off_w1, off_b1, off_w2, off_b2;
+};
+
+__global__ __launch_bounds__(kThreads) void mlp_adam_kernel(AdamArgs a) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= a.P) return;
+  float wv = a.w[p];
+  if (a.slices) {
+    float g = 0.0f;
+    for (uint32_t s = 0; s < a.slices; ++s) g = g + a.part[(uint64_t)s * a.P + p];
+    if (a.grad_out) a.grad_out[p] = g;
+    if (a.update) {
+      const float mv = a.b1 * a.m[p] + (1.0f - a.b1) * g;
+      const float vv = a.b2 * a.v[p] + (1.0f - a.b2) * (g * g);
+      a.m[p] = mv;
+      a.v[p] = vv;
+      const float mh = mv / a.c1, vh = vv / a.c2;
+      wv = wv - a.lr * mh / (sqrtf(vh) + a.eps);
+      a.w[p] = wv;
+    }
+  }
+  if (p >= a.off_w1 && p < a.off_b1) {
+    const uint64_t q = p - a.off_w1, o = q / a.D, i = q % a.D;
+    a.w1b[o * a.Dp + i] = f2bf(wv);
+  } else if (p >= a.off_w2 && p < a.off_b2) {
+    const uint64_t q = p - a.off_w2, o = q / kH, i = q % kH;
+    const uint16_t bv = f2bf(wv);
+    a.w2b[o * kH + i] = bv;
+    a.w2tb[i * kH + o] = bv;
+  }
+}
+
+// Column sums and sums of squares (f64) of rows [b*chunk, (b+1)*chunk) of X.
+__global__ __launch_bounds__(kThreads) void col_moments_kernel(const float* __restrict__ X, uint64_t n,
+                                                               uint32_t D, uint64_t chunk,
+                                                               double* __restrict__ part) {
+  const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = std::min<uint64_t>(n, r0 + chunk);
+  for (uint32_t c = threadIdx.x; c < D; c += kThreads) {
+    double s = 0.0, s2 = 0.0;
+    for (uint64_t r = r0; r < r1; ++r) {
+      const double x = X[r * D + c];
+      s += x;
+      s2 += x * x;
+    }
+    part[((uint64_t)blockIdx.x * D + c) * 2] = s;
+    part[((uint64_t)blockIdx.x * D + c) * 2 + 1] = s2;
+  }
+}
+
+// Mean and standard deviation of each column, reduced in a fixed order.
+void column_stats(const float* X, uint64_t n, uint32_t D, std::vector<double>* mean, std::vector<double>* sd,
+                  hipStream_t s) {
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>(1024, ceil_div(n, 256)));
+  const uint64_t chunk = ceil_div(n, blocks);
+  DevBuf part(blocks * D * 2 * sizeof(double));
+  col_moments_kernel<<<(unsigned)blocks, kThreads, 0, s>>>(X, n, D, chunk, part.as<double>());
+  UMAP_LAUNCH_CHECK();
+  std::vector<double> h(blocks * D * 2);
+  UMAP_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  mean->assign(D, 0.0);
+  sd->assign(D, 0.0);
+  for (uint32_t c = 0; c < D; ++c) {
+    double sum = 0.0, sum2 = 0.0;
+    for (uint64_t b = 0; b < blocks; ++b) {
+      sum += h[(b * D + c) * 2];
+      sum2 += h[(b * D + c) * 2 + 1];
+    }
+    const double mu = sum / (double)n;
+    (*mean)[c] = mu;
+    (*sd)[c] = std::sqrt(std::max(0.0, sum2 / (double)n - mu * mu));
+  }
+}
+
+float host_uniform(uint64_t seed, uint64_t i) {  // [0, 1)
+  return (float)(sm_mix(seed + (i + 1) * kSmGamma) >> 40) * (1.0f / 16777216.0f);
+}
+
+void launch_adam(MlpState* st, uint32_t slices, bool update, float* grad_out, hipStream_t s) {
+  const MlpLayout& L = st->lay;
+  AdamArgs a{};
+  a.w = st->w.as<float>();
+  a.m = st->m.as<float>();
+  a.v = st->v.as<float>();
+  a.part = st->part.as<float>();
+  a.P = L.total;
+  a.slices = slices;
+  a.update = update ? 1 : 0;
+  a.grad_out = grad_out;
+  a.lr = st->cfg.lr;
+  a.b1 = st->cfg.beta1;
+  a.b2 = st->cfg.beta2;
+  a.eps = st->cfg.eps;
+  const double t = (double)st->step;
+  a.c1 = (float)(1.0 - std::pow((double)st->cfg.beta1, t));
+  a.c2 = (float)(1.0 - std::pow((double)st->cfg.beta2, t));
+  a.w1b = st->w1b.as<uint16_t>();
+  a.w2b = st->w2b.as<uint16_t>();
+  a.w2tb = st->w2tb.as<uint16_t>();
+  a.D = (uint32_t)L.D;
+  a.Dp = (uint32_t)L.Dp;
+  a.off_w1 = (uint32_t)L.w1;
+  a.off_b1 = (uint32_t)L.b1;
+  a.off_w2 = (uint32_t)L.w2;
+  a.off_b2 = (uint32_t)L.b2;
+  mlp_adam_kernel<<<(unsigned)ceil_div(L.total, kThreads), kThreads, 0, s>>>(a);
+  UMAP_LAUNCH_CHECK();
+}
+
+RowsArgs rows_args(const MlpState& st, const float* X, uint64_t nrows) {
+  const MlpLayout& L = st.lay;
+  RowsArgs a{};
+  a.X = X;
+  a.D = (uint32_t)L.D;
+  a.Dp = (uint32_t)L.Dp;
+  a.O = (uint32_t)L.O;
+  a.nrows = (uint32_t)nrows;
+  a.xmean = st.xmean.as<float>();
+  a.xinv = st.xinv.as<float>();
+  a.w1b = st.w1b.as<uint16_t>();
+  a.w2b = st.w2b.as<uint16_t>();
+  a.w2tb = st.w2tb.as<uint16_t>();
+  a.params = st.w.as<float>();
+  a.off_b1 = (uint32_t)L.b1;
+  a.off_b2 = (uint32_t)L.b2;
+  a.off_w3 = (uint32_t)L.w3;
+  a.off_b3 = (uint32_t)L.b3;
+  a.ymean = st.ymean.as<float>();
+  a.yscale = st.yscale;
+  a.yinv = 1.0f / st.yscale;
+  a.ldb = (uint32_t)st.bcap;
+  a.xt = st.xt.as<uint16_t>();
+  a.h1t = st.h1t.as<uint16_t>();
+  a.h2t = st.h2t.as<uint16_t>();
+  a.dh1t = st.dh1t.as<uint16_t>();
+  a.dh2t = st.dh2t.as<uint16_t>();
+  a.dyt = st.dyt.as<float>();
+  return a;
+}
+
+// Forward + backward of one batch (rows ids[0..nb) of X/Y, or rows 0..nb-1
+// when ids is null), then the weight-gradient partials.  Returns the number
+// of slices written.
+uint32_t batch_gradients(MlpState* st, const float* X, const float* Y, const uint32_t* ids, uint64_t nb,
+                         float* loss_part, hipStream_t s) {
+  const MlpLayout& L = st->lay;
+  RowsArgs a = rows_args(*st, X, nb);
+  a.ids = ids;
+  a.T = Y;
+  a.dy_scale = (float)(2.0 / ((double)nb * (double)L.O));
+  a.loss_part = loss_part;
+  const unsigned tiles = (unsigned)ceil_div(nb, kR);
+  mlp_rows_kernel<true><<<tiles, kThreads, 0, s>>>(a);
+  UMAP_LAUNCH_CHECK();
+  WgradArgs w{};
+  w.xt = st->xt.as<uint16_t>();
+  w.h1t = st->h1t.as<uint16_t>();
+  w.h2t = st->h2t.as<uint16_t>();
+  w.dh1t = st->dh1t.as<uint16_t>();
+  w.dh2t = st->dh2t.as<uint16_t>();
+  w.dyt = st->dyt.as<float>();
+  w.ldb = (uint32_t)st->bcap;
+  w.kc = tiles * kR;
+  w.D = (uint32_t)L.D;
+  w.Dp = (uint32_t)L.Dp;
+  w.O = (uint32_t)L.O;
+  w.w1_tiles = (uint32_t)(4 * L.Dp / 64);
+  w.part = st->part.as<float>();
+  w.P = L.total;
+  w.off_w1 = (uint32_t)L.w1;
+  w.off_b1 = (uint32_t)L.b1;
+  w.off_w2 = (uint32_t)L.w2;
+  w.off_b2 = (uint32_t)L.b2;
+  w.off_w3 = (uint32_t)L.w3;
+  w.off_b3 = (uint32_t)L.b3;
+  const uint32_t slices = (uint32_t)ceil_div(w.kc, kKS);
+  mlp_wgrad_kernel<<<dim3(16 + w.w1_tiles + 1, slices), kThreads, 0, s>>>(w);
+  UMAP_LAUNCH_CHECK();
+  return slices;
+}
+
+void check_rows(const MlpState& st, uint64_t n) {
+  if (n >= 0xffffffffull) throw Error(Err::InvalidArgument, "MLP row count must be below 2^32");
+  (void)st;
+}
+
+}  // namespace
+
+MlpLayout MlpLayout::make(uint64_t D, uint64_t O) {
+  MlpLayout L;
+  L.D = D;
+  L.O = O;
+  L.Dp = ceil_div(D, 64) * 64;
+  L.w1 = 0;
+  L.b1 = L.w1 + (uint64_t)kH * D;
+  L.w2 = L.b1 + kH;
+  L.b2 = L.w2 + (uint64_t)kH * kH;
+  L.w3 = L.b2 + kH;
+  L.b3 = L.w3 + O * kH;
+  L.total = L.b3 + O;
+  return L;
+}
+
+void mlp_create(const MlpConfig& c, MlpState* st, hipStream_t s) {
+  if (c.in_dim < 1 || c.in_dim > 4096) throw Error(Err::InvalidArgument, "MLP input dimension must be in [1, 4096]");
+  if (c.out_dim < 1 || c.out_dim > (uint64_t)kMlpMaxOut)
+    throw Error(Err::InvalidArgument, "MLP output dimension must be in [1, 8]");
+  if (c.batch < 64 || c.batch > (1u << 20) || c.batch % 64 != 0)
+    throw Error(Err::InvalidArgument, "MLP batch must be a multiple of 64 in [64, 2^20]");
+  if (!(c.lr > 0.0f) || !(c.beta1 >= 0.0f && c.beta1 < 1.0f) || !(c.beta2 >= 0.0f && c.beta2 < 1.0f) ||
+      !(c.eps > 0.0f))
+    throw Error(Err::InvalidArgument, "MLP Adam parameters out of range");
+  st->cfg = c;
+  st->lay = MlpLayout::make(c.in_dim, c.out_dim);
+  const MlpLayout& L = st->lay;
+  st->bcap = c.batch;
+  st->step = 0;
+  st->train_ms = 0.0;
+  st->w.alloc(L.total * 4);
+  st->m.alloc(L.total * 4);
+  st->v.alloc(L.total * 4);
+  st->grad.alloc(L.total * 4);
+  st->w1b.alloc((uint64_t)kH * L.Dp * 2);
+  st->w2b.alloc((uint64_t)kH * kH * 2);
+  st->w2tb.alloc((uint64_t)kH * kH * 2);
+  st->xmean.alloc(L.D * 4);
+  st->xinv.alloc(L.D * 4);
+  st->ymean.alloc(L.O * 4);
+  st->xt.alloc(L.Dp * st->bcap * 2);
+  st->h1t.alloc((uint64_t)kH * st->bcap * 2);
+  st->h2t.alloc((uint64_t)kH * st->bcap * 2);
+  st->dh1t.alloc((uint64_t)kH * st->bcap * 2);
+  st->dh2t.alloc((uint64_t)kH * st->bcap * 2);
+  st->dyt.alloc(L.O * st->bcap * 4);
+  st->part.alloc(ceil_div(st->bcap, kKS) * L.total * 4);
+  st->loss_part.alloc(ceil_div(st->bcap, kR) * 4);
+  st->rows.alloc(st->bcap * 4);
+  // the padded input columns of W1's bf16 copy stay zero
+  UMAP_HIP(hipMemsetAsync(st->w1b.p, 0, st->w1b.bytes, s));
+  // weights: uniform(-sqrt(6/fan_in), sqrt(6/fan_in)) (He, ReLU layers) and
+  // uniform(-sqrt(3/fan_in), ..) for the linear output layer; biases zero
+  std::vector<float> h(L.total, 0.0f);
+  const double l1 = std::sqrt(6.0 / (double)L.D), l2 = std::sqrt(6.0 / kH), l3 = std::sqrt(3.0 / kH);
+  for (uint64_t i = 0; i < (uint64_t)kH * L.D; ++i) h[L.w1 + i] = (float)((2.0 * host_uniform(c.seed, i) - 1.0) * l1);
+  for (uint64_t i = 0; i < (uint64_t)kH * kH; ++i)
+    h[L.w2 + i] = (float)((2.0 * host_uniform(c.seed ^ 0x5555, i) - 1.0) * l2);
+  for (uint64_t i = 0; i < L.O * kH; ++i) h[L.w3 + i] = (float)((2.0 * host_uniform(c.seed ^ 0xaaaa, i) - 1.0) * l3);
+  std::vector<float> zero_d(L.D, 0.0f), one_d(L.D, 1.0f), zero_o(L.O, 0.0f);
+  mlp_set_normalization(st, zero_d.data(), one_d.data(), zero_o.data(), 1.0f, s);
+  mlp_set_params(st, h.data(), s);
+}
+
+void mlp_set_params(MlpState* st, const float* host, hipStream_t s) {
+  const MlpLayout& L = st->lay;
+  UMAP_HIP(hipMemcpyAsync(st->w.p, host, L.total * 4, hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipMemsetAsync(st->m.p, 0, L.total * 4, s));
+  UMAP_HIP(hipMemsetAsync(st->v.p, 0, L.total * 4, s));
+  st->step = 0;
+  launch_adam(st, 0, false, nullptr, s);  // refresh the bf16 copies
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void mlp_get_params(const MlpState& st, float* host, hipStream_t s) {
+  UMAP_HIP(hipMemcpyAsync(host, st.w.p, st.lay.total * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void mlp_set_normalization(MlpState* st, const float* xmean, const float* xinv, const float* ymean, float yscale,
+                           hipStream_t s) {
+  if (!(yscale > 0.0f) || !std::isfinite(yscale)) throw Error(Err::InvalidArgument, "MLP target scale must be positive");
+  const MlpLayout& L = st->lay;
+  UMAP_HIP(hipMemcpyAsync(st->xmean.p, xmean, L.D * 4, hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipMemcpyAsync(st->xinv.p, xinv, L.D * 4, hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipMemcpyAsync(st->ymean.p, ymean, L.O * 4, hipMemcpyHostToDevice, s));
+  st->yscale = yscale;
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void mlp_get_normalization(const MlpState& st, float* xmean, float* xinv, float* ymean, float* yscale,
+                           hipStream_t s) {
+  const MlpLayout& L = st.lay;
+  UMAP_HIP(hipMemcpyAsync(xmean, st.xmean.p, L.D * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipMemcpyAsync(xinv, st.xinv.p, L.D * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipMemcpyAsync(ymean, st.ymean.p, L.O * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  *yscale = st.yscale;
+}
+
+void mlp_fit_normalization(MlpState* st, const float* X, const float* Y, uint64_t n, hipStream_t s) {
+  if (n < 2) throw Error(Err::InvalidArgument, "MLP normalisation needs at least 2 rows");
+  check_rows(*st, n);
+  const MlpLayout& L = st->lay;
+  std::vector<double> mx, sx, my, sy;
+  column_stats(X, n, (uint32_t)L.D, &mx, &sx, s);
+  column_stats(Y, n, (uint32_t)L.O, &my, &sy, s);
+  std::vector<float> xm(L.D), xi(L.D), ym(L.O);
+  for (uint64_t c = 0; c < L.D; ++c) {
+    xm[c] = (float)mx[c];
+    xi[c] = sx[c] > 1e-12 ? (float)(1.0 / sx[c]) : 1.0f;
+  }
+  double var = 0.0;
+  for (uint64_t o = 0; o < L.O; ++o) {
+    ym[o] = (float)my[o];
+    var += sy[o] * sy[o];
+  }
+  const double sc = std::sqrt(var / (double)L.O);
+  mlp_set_normalization(st, xm.data(), xi.data(), ym.data(), sc > 1e-12 ? (float)sc : 1.0f, s);
+}
+
+double mlp_gradients(MlpState* st, const float* X, const float* Y, uint64_t n, float* grad_host, hipStream_t s) {
+  if (n < 1 || n > st->bcap) throw Error(Err::InvalidArgument, "gradient batch must hold 1..batch rows");
+  const uint32_t slices = batch_gradients(st, X, Y, nullptr, n, st->loss_part.as<float>(), s);
+  launch_adam(st, slices, false, st->grad.as<float>(), s);
+  const uint64_t tiles = ceil_div(n, kR);
+  std::vector<float> lp(tiles);
+  UMAP_HIP(hipMemcpyAsync(lp.data(), st->loss_part.p, tiles * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipMemcpyAsync(grad_host, st->grad.p, st->lay.total * 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  double loss = 0.0;
+  for (float v : lp) loss += v;
+  return loss / ((double)n * (double)st->lay.O);
+}
+
+double mlp_train(MlpState* st, const float* X, const float* Y, uint64_t n, uint64_t epochs, hipStream_t s) {
+  if (n < 1) throw Error(Err::InvalidArgument, "MLP training needs at least one row");
+  check_rows(*st, n);
+  const uint64_t B = st->bcap;
+  const uint64_t batches = ceil_div(n, B);
+  const uint64_t tiles_per = ceil_div(B, kR);
+  DevBuf ids(n * 4), losses(batches * tiles_per * 4);
+  std::vector<uint32_t> perm(n);
+  std::vector<float> lp(batches * tiles_per);
+  hipEvent_t e0, e1;
+  UMAP_HIP(hipEventCreate(&e0));
+  UMAP_HIP(hipEventCreate(&e1));
+  double last = 0.0;
+  UMAP_HIP(hipEventRecord(e0, s));
+  for (uint64_t ep = 0; ep < epochs; ++ep) {
+    // Fisher-Yates with the counter RNG of (seed, Adam step count at the epoch start)
+    for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    const uint64_t key = sm_mix(st->cfg.seed + 0x632be59bd9b4e019ULL + st->step * kSmGamma2);
+    for (uint64_t i = n - 1; i > 0; --i) {
+      const uint64_t j = rng_index(key, i, (uint32_t)(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+    UMAP_HIP(hipMemcpyAsync(ids.p, perm.data(), n * 4, hipMemcpyHostToDevice, s));
+    for (uint64_t b = 0; b < batches; ++b) {
+      const uint64_t nb = std::min<uint64_t>(B, n - b * B);
+      const uint32_t slices = batch_gradients(st, X, Y, ids.as<uint32_t>() + b * B, nb,
+                                              losses.as<float>() + b * tiles_per, s);
+      ++st->step;
+      launch_adam(st, slices, true, nullptr, s);
+    }
+    if (ep + 1 == epochs) {
+      UMAP_HIP(hipMemcpyAsync(lp.data(), losses.p, lp.size() * 4, hipMemcpyDeviceToHost, s));
+      UMAP_HIP(hipStreamSynchronize(s));
+      double tot = 0.0;
+      for (uint64_t b = 0; b < batches; ++b) {
+        const uint64_t nb = std::min<uint64_t>(B, n - b * B);
+        for (uint64_t tt = 0; tt < ceil_div(nb, kR); ++tt) tot += lp[b * tiles_per + tt];
+      }
+      last = tot / ((double)n * (double)st->lay.O);
+    } else {
+      // the host permutation buffer is rewritten next epoch
+      UMAP_HIP(hipStreamSynchronize(s));
+    }
+  }
+  UMAP_HIP(hipEventRecord(e1, s));
+  UMAP_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  UMAP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  st->train_ms += ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return last;
+}
+
+void mlp_predict(const MlpState& st, const float* X, uint64_t n, float* Y, hipStream_t s) {
+  check_rows(st, n);
+  if (n == 0) return;
+  RowsArgs a = rows_args(st, X, n);
+  a.Y = Y;
+  mlp_rows_kernel<false><<<(unsigned)ceil_div(n, kR), kThreads, 0, s>>>(a);
+  UMAP_LAUNCH_CHECK();
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace umapgpu

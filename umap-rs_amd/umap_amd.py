@@ -59,6 +59,11 @@ class _Config(C.Structure):
 DISTANCE_FN = C.CFUNCTYPE(f32, C.POINTER(f32), C.POINTER(f32), u64, C.POINTER(f32), P)
 
 
+class _MlpConfig(C.Structure):
+    _fields_ = [("in_dim", u64), ("out_dim", u64), ("batch", u64), ("learning_rate", f32), ("beta1", f32),
+                ("beta2", f32), ("eps", f32), ("seed", u64)]
+
+
 class _CeTerms(C.Structure):
     _fields_ = [("total", f64), ("attract", f64), ("repulse_sum", f64), ("edges", u64), ("pairs", u64)]
 
@@ -134,6 +139,19 @@ def lib():
             "umapgpu_optimizer_fuzzy_cross_entropy": ([P, u64, u64, C.POINTER(_CeTerms)], i32),
             "umapgpu_knn_query_device": ([P, u64, u64, P, u64, u64, P], i32),
             "umapgpu_trustworthiness_penalty_device": ([P, u64, u64, P, u64, P, u64, C.POINTER(f64)], i32),
+            "umapgpu_mlp_config_default": ([C.POINTER(_MlpConfig)], None),
+            "umapgpu_mlp_create": ([C.POINTER(_MlpConfig), C.POINTER(P)], i32),
+            "umapgpu_mlp_destroy": ([P], None),
+            "umapgpu_mlp_param_count": ([P], u64),
+            "umapgpu_mlp_get_params": ([P, P], i32),
+            "umapgpu_mlp_set_params": ([P, P], i32),
+            "umapgpu_mlp_fit_normalization": ([P, P, P, u64], i32),
+            "umapgpu_mlp_set_normalization": ([P, P, P, P, f32], i32),
+            "umapgpu_mlp_get_normalization": ([P, P, P, P, C.POINTER(f32)], i32),
+            "umapgpu_mlp_gradients": ([P, P, P, u64, P, C.POINTER(f64)], i32),
+            "umapgpu_mlp_train": ([P, P, P, u64, u64, C.POINTER(f64)], i32),
+            "umapgpu_mlp_predict": ([P, P, u64, P], i32),
+            "umapgpu_mlp_stats": ([P, C.POINTER(u64), C.POINTER(f64)], i32),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
             "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
@@ -735,6 +753,91 @@ def knn_bruteforce_device(data: DeviceArray, k: int):
     dst = DeviceArray((n, k), np.float32)
     _check(lib().umapgpu_knn_bruteforce_device(P(data.ptr), n, dim, k, P(idx.ptr), P(dst.ptr)))
     return idx, dst
+
+
+@dataclasses.dataclass
+class MlpConfig:
+    """Learned-initialisation network: in_dim -> 256 -> 256 -> out_dim (ReLU)."""
+    in_dim: int
+    out_dim: int = 2
+    batch: int = 4096
+    learning_rate: float = 1e-3
+    beta1: float = 0.9
+    beta2: float = 0.999
+    eps: float = 1e-8
+    seed: int = 0
+
+    def _to_c(self) -> _MlpConfig:
+        return _MlpConfig(self.in_dim, self.out_dim, self.batch, self.learning_rate, self.beta1, self.beta2,
+                          self.eps, self.seed)
+
+
+class Mlp(_Handle):
+    """Device MLP that predicts an initial layout from the input features
+    (bf16 MFMA forward/backward, f32 master weights, Adam)."""
+    _destroy = "umapgpu_mlp_destroy"
+
+    def __init__(self, config: MlpConfig):
+        out = P()
+        c = config._to_c()
+        _check(lib().umapgpu_mlp_create(C.byref(c), C.byref(out)))
+        super().__init__(out)
+        self.config = config
+
+    def param_count(self) -> int:
+        return int(lib().umapgpu_mlp_param_count(self._handle()))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.param_count(), np.float32)
+        _check(lib().umapgpu_mlp_get_params(self._handle(), _ptr(out)))
+        return out
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.param_count():
+            raise ValueError(f"expected {self.param_count()} parameters, got {p.size}")
+        _check(lib().umapgpu_mlp_set_params(self._handle(), _ptr(p)))
+
+    def fit_normalization(self, x: DeviceArray, y: DeviceArray, n: Optional[int] = None) -> None:
+        _check(lib().umapgpu_mlp_fit_normalization(self._handle(), P(x.ptr), P(y.ptr), n or x.shape[0]))
+
+    def set_normalization(self, x_mean, x_inv_std, y_mean, y_scale: float) -> None:
+        xm = np.ascontiguousarray(x_mean, np.float32)
+        xi = np.ascontiguousarray(x_inv_std, np.float32)
+        ym = np.ascontiguousarray(y_mean, np.float32)
+        _check(lib().umapgpu_mlp_set_normalization(self._handle(), _ptr(xm), _ptr(xi), _ptr(ym), float(y_scale)))
+
+    def get_normalization(self):
+        xm = np.zeros(self.config.in_dim, np.float32)
+        xi = np.zeros(self.config.in_dim, np.float32)
+        ym = np.zeros(self.config.out_dim, np.float32)
+        ys = f32()
+        _check(lib().umapgpu_mlp_get_normalization(self._handle(), _ptr(xm), _ptr(xi), _ptr(ym), C.byref(ys)))
+        return xm, xi, ym, float(ys.value)
+
+    def gradients(self, x: DeviceArray, y: DeviceArray, n: Optional[int] = None):
+        g = np.zeros(self.param_count(), np.float32)
+        loss = f64()
+        _check(lib().umapgpu_mlp_gradients(self._handle(), P(x.ptr), P(y.ptr), n or x.shape[0], _ptr(g),
+                                           C.byref(loss)))
+        return loss.value, g
+
+    def train(self, x: DeviceArray, y: DeviceArray, epochs: int, n: Optional[int] = None) -> float:
+        loss = f64()
+        _check(lib().umapgpu_mlp_train(self._handle(), P(x.ptr), P(y.ptr), n or x.shape[0], epochs,
+                                       C.byref(loss)))
+        return loss.value
+
+    def predict(self, x: DeviceArray, out: Optional[DeviceArray] = None) -> DeviceArray:
+        n = x.shape[0]
+        out = out or DeviceArray((n, self.config.out_dim), np.float32)
+        _check(lib().umapgpu_mlp_predict(self._handle(), P(x.ptr), n, P(out.ptr)))
+        return out
+
+    def stats(self):
+        steps, ms = u64(), f64()
+        _check(lib().umapgpu_mlp_stats(self._handle(), C.byref(steps), C.byref(ms)))
+        return steps.value, ms.value
 
 
 def knn_query_device(data: DeviceArray, queries: DeviceArray, k: int) -> DeviceArray:
