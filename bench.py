@@ -13,6 +13,12 @@ as graph_build_s).  value = positive edge-updates per second of the timed
 steps (an edge-update is one activation of optimize_layout_euclidean.rs:277-344
 with its negative samples; the count is the RNG-free f64 schedule).
 
+wall_to_target compares the learned initialisation with the plain fit: the
+plain run (random init, 500 epochs) defines the target loss 1.01 x its final
+fuzzy cross-entropy; each run's clock covers Optimizer::new and the SGD calls
+(plus, for the learned run, the subsample fit, MLP training and prediction),
+while the loss evaluations between SGD chunks are excluded.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -55,6 +61,95 @@ def bytes_per_update(dim: int, nsr: int) -> int:
     return 8 + 16 * dim + 4 * dim * nsr
 
 
+def ce_trace(U, manifold, init_ptr, n, epochs, cfg, every, seed, n_random=200_000):
+    """A fresh optimizer from `init_ptr`, stepped in chunks of `every` epochs;
+    the fuzzy cross-entropy after each chunk is evaluated off the clock.
+    Returns ([(epoch, seconds, ce)], optimizer)."""
+    m = manifold.clone()
+    U.synchronize()
+    t = time.perf_counter()
+    opt = U.Optimizer.new_device(m, init_ptr, n, 2, epochs, cfg, U.MetricType.Euclidean, seed=seed)
+    U.synchronize()
+    clock = time.perf_counter() - t
+    trace = [(0, clock, opt.fuzzy_cross_entropy(n_random, 3)["total"])]
+    done = 0
+    while done < epochs:
+        c = min(every, epochs - done)
+        t = time.perf_counter()
+        opt.step_epochs(c)
+        U.synchronize()
+        clock += time.perf_counter() - t
+        done += c
+        trace.append((done, clock, opt.fuzzy_cross_entropy(n_random, 3)["total"]))
+    return trace, opt
+
+
+def first_reach(trace, target):
+    for ep, t, ce in trace:
+        if ce <= target:
+            return ep, t
+    return None, None
+
+
+def wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs):
+    import umap_eval as E
+    knn_host = d_idx.to_host()
+    plain, popt = ce_trace(U, manifold, d_init.ptr, n, epochs, cfg, 10, seed=7)
+    target = 1.01 * plain[-1][2]
+    p_ep, p_t = first_reach(plain, target)
+    p_emb = popt.embedding_device()
+    p_kp = E.knn_preservation_gpu(knn_host, p_emb, k=15)
+    p_tw = E.trustworthiness_gpu(dx, p_emb, k=15)
+    log(f"plain: final CE {plain[-1][2]:.6g} after {plain[-1][1]:.3f}s; target {target:.6g} reached at epoch "
+        f"{p_ep} ({p_t:.3f}s); kNN preservation {p_kp:.4f}, trustworthiness {p_tw:.4f}")
+    # PCA of all rows as the init of a plain run: the non-learned comparison point
+    U.synchronize()
+    t = time.perf_counter()
+    pca_init, _ = U.pca_device(dx, 2)
+    U.synchronize()
+    pca_s = time.perf_counter() - t
+    pca, qopt = ce_trace(U, manifold, pca_init.ptr, n, epochs, cfg, 10, seed=9)
+    q_ep, q_t = first_reach(pca, target)
+    q_kp = E.knn_preservation_gpu(knn_host, qopt.embedding_device(), k=15)
+    del qopt
+    pca_init.free()
+    log(f"pca init: final CE {pca[-1][2]:.6g}; target reached at epoch {q_ep}; kNN preservation {q_kp:.4f}")
+    li = U.LearnedInitConfig(n_subsample=min(args.subsample, n), subsample_epochs=500, mlp_epochs=args.mlp_epochs,
+                             seed=0)
+    init_l, info = U.learned_init(cfg, dx, li)
+    log(f"learned init: {info}")
+    learned, lopt = ce_trace(U, manifold, init_l.ptr, n, args.learned_epochs, cfg, 5, seed=8)
+    l_ep, l_t = first_reach(learned, target)
+    l_emb = lopt.embedding_device()
+    l_kp = E.knn_preservation_gpu(knn_host, l_emb, k=15)
+    l_tw = E.trustworthiness_gpu(dx, l_emb, k=15)
+    init_ce = learned[0][2]
+    log(f"learned: CE at init {init_ce:.6g}, final {learned[-1][2]:.6g} after {learned[-1][1]:.3f}s SGD; target "
+        f"reached at epoch {l_ep}; kNN preservation {l_kp:.4f}, trustworthiness {l_tw:.4f}")
+    total_l = None if l_t is None else info["total_s"] + l_t
+    accepted = bool(learned[-1][2] <= target and l_kp >= p_kp - 0.01)
+    return {
+        "target_ce": round(target, 2),
+        "plain": {"epochs": epochs, "final_ce": round(plain[-1][2], 2), "full_run_s": round(plain[-1][1], 4),
+                  "epoch_at_target": p_ep, "time_to_target_s": round(p_t, 4), "knn_preservation": round(p_kp, 4),
+                  "trustworthiness": round(p_tw, 4), "ce_every_10": [round(c, 1) for _, _, c in plain]},
+        "learned": {"epochs": args.learned_epochs, "init_ce": round(init_ce, 2), "final_ce": round(learned[-1][2], 2),
+                    "sgd_s": round(learned[-1][1], 4), "epoch_at_target": l_ep,
+                    "time_to_target_s": None if total_l is None else round(total_l, 4),
+                    "init_stages_s": {k: round(v, 4) for k, v in info.items() if k.endswith("_s")},
+                    "mlp_steps": info["mlp_steps"], "mlp_final_loss": round(info["mlp_loss"], 5),
+                    "knn_preservation": round(l_kp, 4), "trustworthiness": round(l_tw, 4),
+                    "ce_every_5": [round(c, 1) for _, _, c in learned]},
+        "pca_init_plain": {"epochs": epochs, "pca_s": round(pca_s, 4), "final_ce": round(pca[-1][2], 2),
+                           "epoch_at_target": q_ep,
+                           "time_to_target_s": None if q_t is None else round(pca_s + q_t, 4),
+                           "knn_preservation": round(q_kp, 4)},
+        "accepted": accepted,
+        "time_ratio_vs_plain_full": None if total_l is None else round(total_l / plain[-1][1], 4),
+        "time_ratio_vs_plain_to_target": None if total_l is None else round(total_l / p_t, 4),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +163,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's SGD sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-quality", action="store_true")
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned-init wall-to-target comparison")
+    ap.add_argument("--learned-epochs", type=int, default=100)
+    ap.add_argument("--subsample", type=int, default=200_000)
+    ap.add_argument("--mlp-epochs", type=int, default=30)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scales profiler traffic counters)")
     args = ap.parse_args()
@@ -100,7 +199,6 @@ def main() -> int:
     U.synchronize()
     knn_s = time.perf_counter() - t
     log(f"exact kNN k={k} on GPU: {knn_s:.2f}s")
-    dx.free()
 
     # graph build (learn_manifold from resident kNN)
     U.synchronize()
@@ -207,6 +305,9 @@ def main() -> int:
         t = time.perf_counter()
         out["final_fuzzy_ce"] = round(E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), y, a, b), 1)
         log(f"final fuzzy CE {out['final_fuzzy_ce']:.4g} ({time.perf_counter() - t:.1f}s)")
+
+    if not args.skip_learned:
+        out["wall_to_target"] = wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs)
 
     if not args.skip_cpu:
         import oracle as O

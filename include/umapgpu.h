@@ -307,6 +307,15 @@ UMAPGPU_API umapgpu_status umapgpu_mlp_predict(const umapgpu_mlp* m, const float
                                                float* y_device);
 UMAPGPU_API umapgpu_status umapgpu_mlp_stats(const umapgpu_mlp* m, uint64_t* adam_steps, double* train_ms);
 
+/* Principal-component initial layout (no reference counterpart; the
+ * reference's examples compute a PCA init on the CPU before calling fit,
+ * examples/src/bin/mnist_demo.rs:247-302).  y (n x out_dim, device) = data
+ * (n x D, device, D <= 384) centred and projected on its top out_dim
+ * principal axes (each axis signed so that its largest entry is positive);
+ * variances[out_dim] (optional) receives the axes' variances. */
+UMAPGPU_API umapgpu_status umapgpu_pca_device(const float* data_device, uint64_t n, uint64_t dim, uint64_t out_dim,
+                                              float* y_device, double* variances);
+
 /* ---- evaluators (no reference counterpart: the reference defines no quality
  * measure).  Used to judge embeddings and to time wall-to-target-loss. ---- */
 typedef struct umapgpu_ce_terms {
@@ -347,6 +356,12 @@ UMAPGPU_API umapgpu_status umapgpu_trustworthiness_penalty_device(const float* d
                                                                   uint64_t dim, const uint32_t* queries_device,
                                                                   uint64_t nq, const uint32_t* nbrs_device,
                                                                   uint64_t k, double* penalty);
+/* dst (m x dim) row r = src (n x dim) row rows[r] (device arrays); rows
+ * outside [0, n) give NaN rows.  Used to draw the learned-init subsample. */
+UMAPGPU_API umapgpu_status umapgpu_gather_rows_device(const float* src_device, uint64_t n, uint64_t dim,
+                                                      const uint32_t* rows_device, uint64_t m, float* dst_device);
+/* Device-to-device copy of `bytes`. */
+UMAPGPU_API umapgpu_status umapgpu_memcpy_dtod(void* dst_device, const void* src_device, uint64_t bytes);
 /* Calibration hook: a plain streaming copy of `bytes` (a multiple of 16),
  * used to scale hardware traffic counters into bytes. */
 UMAPGPU_API umapgpu_status umapgpu_debug_copy_device(const void* src_device, void* dst_device, uint64_t bytes);

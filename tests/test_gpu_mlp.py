@@ -135,3 +135,24 @@ def test_mlp_config_errors(gpu):
         U.Mlp(U.MlpConfig(in_dim=8, batch=100))
     with pytest.raises(U.UmapPanic, match="output dimension"):
         U.Mlp(U.MlpConfig(in_dim=8, out_dim=9))
+
+
+def test_gpu_pca_matches_numpy(gpu):
+    U = gpu
+    rng = np.random.default_rng(8)
+    n, D = 20000, 40
+    scales = np.linspace(3.0, 0.2, D)
+    basis = np.linalg.qr(rng.normal(size=(D, D)))[0]
+    x = ((rng.normal(size=(n, D)) * scales) @ basis.T + rng.normal(0, 4, D)).astype(np.float32)
+    dx = U.DeviceArray.from_host(x)
+    y, var = U.pca_device(dx, 2)
+    y = y.to_host()
+    xc = x.astype(np.float64) - x.astype(np.float64).mean(0)
+    w, v = np.linalg.eigh(xc.T @ xc / n)
+    order = np.argsort(w)[::-1][:2]
+    comp = v[:, order].T
+    comp *= np.sign(comp[np.arange(2), np.abs(comp).argmax(1)])[:, None]
+    want = xc @ comp.T
+    np.testing.assert_allclose(var, w[order], rtol=1e-5)
+    np.testing.assert_allclose(y, want, rtol=0, atol=1e-4 * np.abs(want).max())
+

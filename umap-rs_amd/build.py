@@ -25,7 +25,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.environ.get("UMAPGPU_BUILD_DIR") or os.path.join(HERE, "_build")
 LIB = os.environ.get("UMAPGPU_LIB_OUT") or os.path.join(HERE, "libumapgpu.so")
-SOURCES = ["api.hip", "graph.hip", "optimize.hip", "primitives.hip", "knn.hip", "evaluate.hip", "mlp.hip"]
+SOURCES = ["api.hip", "graph.hip", "optimize.hip", "primitives.hip", "knn.hip", "evaluate.hip", "mlp.hip", "pca.hip"]
 ARCH = os.environ.get("UMAPGPU_ARCH", "gfx950")
 
 COMMON = [

@@ -14,6 +14,7 @@
 #include "graph.hpp"
 #include "knn.hpp"
 #include "mlp.hpp"
+#include "pca.hpp"
 #include "optimize.hpp"
 #include "primitives.hpp"
 
@@ -988,6 +989,17 @@ umapgpu_status umapgpu_mlp_stats(const umapgpu_mlp* m, uint64_t* steps, double* 
   });
 }
 
+umapgpu_status umapgpu_pca_device(const float* x, uint64_t n, uint64_t dim, uint64_t out_dim, float* y,
+                                  double* variances) {
+  return guarded([&] {
+    ensure_device();
+    if (!x || !y) throw Error(Err::InvalidArgument, "null argument");
+    PcaResult r = pca_project(x, n, dim, out_dim, y, stream());
+    if (variances)
+      for (uint64_t i = 0; i < out_dim; ++i) variances[i] = r.variances[i];
+  });
+}
+
 namespace {
 void ce_out(const CeTerms& t, umapgpu_ce_terms* out) {
   out->total = t.total;
@@ -1048,6 +1060,24 @@ umapgpu_status umapgpu_trustworthiness_penalty_device(const float* data, uint64_
     ensure_device();
     if (!penalty) throw Error(Err::InvalidArgument, "null output");
     *penalty = trustworthiness_penalty(data, n, dim, queries, nq, nbrs, k, stream());
+  });
+}
+
+umapgpu_status umapgpu_gather_rows_device(const float* src, uint64_t n, uint64_t dim, const uint32_t* rows,
+                                          uint64_t m, float* dst) {
+  return guarded([&] {
+    ensure_device();
+    if (m && dim && (!src || !rows || !dst)) throw Error(Err::InvalidArgument, "null argument");
+    gather_rows_f32(src, n, dim, rows, m, dst, stream());
+    UMAP_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+umapgpu_status umapgpu_memcpy_dtod(void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    ensure_device();
+    if (bytes) UMAP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+    UMAP_HIP(hipStreamSynchronize(stream()));
   });
 }
 

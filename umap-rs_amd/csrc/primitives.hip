@@ -230,7 +230,26 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
+// dst row r = src row rows[r] (dim floats each); rows outside [0, n) give NaN rows.
+__global__ void gather_rows_kernel(const float* __restrict__ src, uint64_t n, uint64_t dim,
+                                   const uint32_t* __restrict__ rows, uint64_t m, float* __restrict__ dst) {
+  const uint64_t total = m * dim;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = e / dim, c = e % dim;
+    const uint32_t row = rows[r];
+    dst[e] = row < n ? src[(uint64_t)row * dim + c] : __builtin_nanf("");
+  }
+}
+
 }  // namespace
+
+void gather_rows_f32(const float* src, uint64_t n, uint64_t dim, const uint32_t* rows, uint64_t m, float* dst,
+                     hipStream_t s) {
+  if (m == 0 || dim == 0) return;
+  unsigned nb = (unsigned)std::min<uint64_t>(ceil_div(m * dim, 256), 65535);
+  gather_rows_kernel<<<nb, 256, 0, s>>>(src, n, dim, rows, m, dst);
+  UMAP_LAUNCH_CHECK();
+}
 
 void exclusive_scan(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s) { exclusive_scan_impl(in, out, n, s); }
 void exclusive_scan(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t s) { exclusive_scan_impl(in, out, n, s); }

@@ -21,4 +21,8 @@ float reduce_min_f32(const float* x, uint64_t n, uint64_t stride, float init, hi
 
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
+// dst (m x dim) row r = src (n x dim) row rows[r].
+void gather_rows_f32(const float* src, uint64_t n, uint64_t dim, const uint32_t* rows, uint64_t m, float* dst,
+                     hipStream_t s);
+
 }  // namespace umapgpu

@@ -152,6 +152,9 @@ def lib():
             "umapgpu_mlp_train": ([P, P, P, u64, u64, C.POINTER(f64)], i32),
             "umapgpu_mlp_predict": ([P, P, u64, P], i32),
             "umapgpu_mlp_stats": ([P, C.POINTER(u64), C.POINTER(f64)], i32),
+            "umapgpu_gather_rows_device": ([P, u64, u64, P, u64, P], i32),
+            "umapgpu_memcpy_dtod": ([P, P, u64], i32),
+            "umapgpu_pca_device": ([P, u64, u64, u64, P, P], i32),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
             "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
@@ -566,6 +569,11 @@ class Optimizer(_Handle):
     def embedding_device_ptr(self) -> int:
         return lib().umapgpu_optimizer_embedding_device(self._handle())
 
+    def embedding_device(self) -> "DeviceArray":
+        """Non-owning device view of the current embedding (valid while the optimizer lives)."""
+        _, nv, d = self.info()
+        return DeviceArray.view(self.embedding_device_ptr(), (nv, d), np.float32, owner=self)
+
     def manifold(self) -> LearnedManifold:
         return LearnedManifold._borrowed(lib().umapgpu_optimizer_manifold(self._handle()), self)
 
@@ -734,17 +742,50 @@ class DeviceArray:
             _check(lib().umapgpu_memcpy_dtoh(_ptr(out), P(self.ptr), self.nbytes))
         return out
 
+    @staticmethod
+    def view(ptr: int, shape, dtype, owner=None) -> "DeviceArray":
+        """A non-owning view of device memory owned elsewhere (kept alive by `owner`)."""
+        d = DeviceArray.__new__(DeviceArray)
+        d.shape = tuple(int(s) for s in shape)
+        d.dtype = np.dtype(dtype)
+        d.nbytes = int(np.prod(d.shape)) * d.dtype.itemsize
+        d.ptr = ptr
+        d._view_of = owner
+        return d
+
+    def copy(self) -> "DeviceArray":
+        d = DeviceArray(self.shape, self.dtype)
+        _check(lib().umapgpu_memcpy_dtod(P(d.ptr), P(self.ptr), self.nbytes))
+        return d
+
     def free(self):
-        if self.ptr:
+        if self.ptr and not hasattr(self, "_view_of"):
             _check(lib().umapgpu_device_free(P(self.ptr)))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
-            if self.ptr and _lib is not None:
+            if self.ptr and _lib is not None and not hasattr(self, "_view_of"):
                 _lib.umapgpu_device_free(P(self.ptr))
         except Exception:
             pass
+
+
+def pca_device(x: DeviceArray, out_dim: int = 2):
+    """Principal-component projection of device rows; returns (DeviceArray n x out_dim, variances)."""
+    n, dim = x.shape
+    out = DeviceArray((n, out_dim), np.float32)
+    var = np.zeros(out_dim, np.float64)
+    _check(lib().umapgpu_pca_device(P(x.ptr), n, dim, out_dim, P(out.ptr), _ptr(var)))
+    return out, var
+
+
+def gather_rows_device(src: DeviceArray, rows: DeviceArray) -> DeviceArray:
+    n, dim = src.shape
+    m = rows.shape[0]
+    out = DeviceArray((m, dim), np.float32)
+    _check(lib().umapgpu_gather_rows_device(P(src.ptr), n, dim, P(rows.ptr), m, P(out.ptr)))
+    return out
 
 
 def knn_bruteforce_device(data: DeviceArray, k: int):
@@ -891,3 +932,78 @@ def debug_expf(x: np.ndarray) -> np.ndarray:
     dy = DeviceArray(x.shape, np.float32)
     _check(lib().umapgpu_debug_expf_device(P(dx.ptr), P(dy.ptr), x.size))
     return dy.to_host()
+
+
+# ---------------------------------------------------------------------------
+# learned initialisation
+
+@dataclasses.dataclass
+class LearnedInitConfig:
+    """Subsample fit + MLP used to predict an initial layout for every point."""
+    n_subsample: int = 200_000
+    subsample_epochs: int = 500
+    mlp_epochs: int = 30
+    batch: int = 4096
+    learning_rate: float = 1e-3
+    seed: int = 0
+    subsample_init: str = "pca"  # init of the subsample fit: "pca" or "random" (uniform [-10, 10])
+
+
+def learned_init(config: "UmapConfig", x: DeviceArray, li: Optional[LearnedInitConfig] = None):
+    """Predict an initial layout for the n rows of x (device, n x D).
+
+    A random subsample of li.n_subsample rows gets an exact kNN graph and a
+    plain fit of li.subsample_epochs epochs, started from the subsample's
+    principal components (li.subsample_init "pca") or uniform [-10, 10]
+    ("random"); an MLP (D -> 256 -> 256 -> n_components) is trained on that
+    embedding and predicts every row.  Returns (init DeviceArray n x n_components, info)
+    with the stage wall times in seconds."""
+    import copy
+    import time
+    li = li or LearnedInitConfig()
+    n, D = x.shape
+    dim = config.n_components
+    k = config.graph.n_neighbors
+    rng = np.random.default_rng(li.seed)
+    m = min(li.n_subsample, n)
+    info = {"n_subsample": m}
+    synchronize()
+    t0 = time.perf_counter()
+    sub = np.sort(rng.choice(n, m, replace=False)).astype(np.uint32) if m < n else np.arange(n, dtype=np.uint32)
+    d_sub = DeviceArray.from_host(sub)
+    xs = gather_rows_device(x, d_sub)
+    d_sub.free()
+    idx, dst = knn_bruteforce_device(xs, k)
+    synchronize()
+    t1 = time.perf_counter()
+    cfg = copy.deepcopy(config)
+    cfg.optimization.n_epochs = li.subsample_epochs
+    man = learn_manifold_device(cfg, idx, dst)
+    idx.free()
+    dst.free()
+    if li.subsample_init == "pca":
+        init_dev, _ = pca_device(xs, dim)
+    elif li.subsample_init == "random":
+        init_dev = DeviceArray.from_host(rng.uniform(-10, 10, (m, dim)).astype(np.float32))
+    else:
+        raise ValueError(f"unknown subsample_init {li.subsample_init!r}")
+    opt = Optimizer.new_device(man, init_dev.ptr, m, dim, li.subsample_epochs, cfg, MetricType.Euclidean,
+                               seed=li.seed + 1)
+    init_dev.free()
+    opt.step_epochs(li.subsample_epochs)
+    target = opt.embedding_device()
+    synchronize()
+    t2 = time.perf_counter()
+    mlp = Mlp(MlpConfig(in_dim=D, out_dim=dim, batch=li.batch, learning_rate=li.learning_rate, seed=li.seed + 2))
+    mlp.fit_normalization(xs, target)
+    info["mlp_loss"] = mlp.train(xs, target, li.mlp_epochs)
+    synchronize()
+    t3 = time.perf_counter()
+    out = mlp.predict(x)
+    synchronize()
+    t4 = time.perf_counter()
+    xs.free()
+    info.update({"subsample_knn_s": t1 - t0, "subsample_fit_s": t2 - t1, "mlp_train_s": t3 - t2,
+                 "predict_s": t4 - t3, "total_s": t4 - t0, "mlp_steps": mlp.stats()[0],
+                 "mlp_train_ms_events": mlp.stats()[1]})
+    return out, info
