@@ -61,7 +61,7 @@ def bytes_per_update(dim: int, nsr: int) -> int:
     return 8 + 16 * dim + 4 * dim * nsr
 
 
-def ce_trace(U, manifold, init_ptr, n, epochs, cfg, every, seed, n_random=200_000):
+def ce_trace(U, manifold, init_ptr, n, epochs, cfg, every, seed, semantics=0, n_random=200_000):
     """A fresh optimizer from `init_ptr`, stepped in chunks of `every` epochs;
     the fuzzy cross-entropy after each chunk is evaluated off the clock.
     Returns ([(epoch, seconds, ce)], optimizer)."""
@@ -69,6 +69,7 @@ def ce_trace(U, manifold, init_ptr, n, epochs, cfg, every, seed, n_random=200_00
     U.synchronize()
     t = time.perf_counter()
     opt = U.Optimizer.new_device(m, init_ptr, n, 2, epochs, cfg, U.MetricType.Euclidean, seed=seed)
+    opt.set_semantics(semantics)
     U.synchronize()
     clock = time.perf_counter() - t
     trace = [(0, clock, opt.fuzzy_cross_entropy(n_random, 3)["total"])]
@@ -91,10 +92,10 @@ def first_reach(trace, target):
     return None, None
 
 
-def wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs):
+def wall_to_target(U, args, cfg, manifold, d_init, dx, knn_host, n, epochs, semantics):
     import umap_eval as E
-    knn_host = d_idx.to_host()
-    plain, popt = ce_trace(U, manifold, d_init.ptr, n, epochs, cfg, 10, seed=7)
+    log(f"wall-to-target, SGD semantics {semantics}")
+    plain, popt = ce_trace(U, manifold, d_init.ptr, n, epochs, cfg, 10, seed=7, semantics=semantics)
     target = 1.01 * plain[-1][2]
     p_ep, p_t = first_reach(plain, target)
     p_emb = popt.embedding_device()
@@ -108,17 +109,17 @@ def wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs):
     pca_init, _ = U.pca_device(dx, 2)
     U.synchronize()
     pca_s = time.perf_counter() - t
-    pca, qopt = ce_trace(U, manifold, pca_init.ptr, n, epochs, cfg, 10, seed=9)
+    pca, qopt = ce_trace(U, manifold, pca_init.ptr, n, epochs, cfg, 10, seed=9, semantics=semantics)
     q_ep, q_t = first_reach(pca, target)
     q_kp = E.knn_preservation_gpu(knn_host, qopt.embedding_device(), k=15)
     del qopt
     pca_init.free()
     log(f"pca init: final CE {pca[-1][2]:.6g}; target reached at epoch {q_ep}; kNN preservation {q_kp:.4f}")
     li = U.LearnedInitConfig(n_subsample=min(args.subsample, n), subsample_epochs=500, mlp_epochs=args.mlp_epochs,
-                             seed=0)
+                             seed=0, semantics=semantics)
     init_l, info = U.learned_init(cfg, dx, li)
     log(f"learned init: {info}")
-    learned, lopt = ce_trace(U, manifold, init_l.ptr, n, args.learned_epochs, cfg, 5, seed=8)
+    learned, lopt = ce_trace(U, manifold, init_l.ptr, n, args.learned_epochs, cfg, 5, seed=8, semantics=semantics)
     l_ep, l_t = first_reach(learned, target)
     l_emb = lopt.embedding_device()
     l_kp = E.knn_preservation_gpu(knn_host, l_emb, k=15)
@@ -129,6 +130,8 @@ def wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs):
     total_l = None if l_t is None else info["total_s"] + l_t
     accepted = bool(learned[-1][2] <= target and l_kp >= p_kp - 0.01)
     return {
+        "sgd_semantics": {0: "reference (per-call tail snapshot, a*d2^b*d2+1 denominators)",
+                          3: "opt-in umap-learn (one shared array, a*d2^b+1 denominators)"}[semantics],
         "target_ce": round(target, 2),
         "plain": {"epochs": epochs, "final_ce": round(plain[-1][2], 2), "full_run_s": round(plain[-1][1], 4),
                   "epoch_at_target": p_ep, "time_to_target_s": round(p_t, 4), "knn_preservation": round(p_kp, 4),
@@ -307,7 +310,10 @@ def main() -> int:
         log(f"final fuzzy CE {out['final_fuzzy_ce']:.4g} ({time.perf_counter() - t:.1f}s)")
 
     if not args.skip_learned:
-        out["wall_to_target"] = wall_to_target(U, args, cfg, manifold, d_init, dx, d_idx, n, epochs)
+        knn_host = d_idx.to_host()
+        out["wall_to_target"] = wall_to_target(U, args, cfg, manifold, d_init, dx, knn_host, n, epochs, 0)
+        out["wall_to_target_umap_learn_semantics"] = wall_to_target(U, args, cfg, manifold, d_init, dx, knn_host, n,
+                                                                    epochs, 3)
 
     if not args.skip_cpu:
         import oracle as O

@@ -175,6 +175,21 @@ UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs(umapgpu_optimizer* opt,
 UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* opt, uint64_t n,
                                                                  umapgpu_distance_fn distance, void* user);
 UMAPGPU_API umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* opt, int32_t mode);
+/* Opt-in SGD semantics (no reference counterpart; default 0 = the
+ * reference).  Bits:
+ *  UMAPGPU_SEMANTICS_SHARED_BUFFER: head and tail updates go to one array,
+ *    as in umap-learn, instead of the tail snapshot the reference takes at
+ *    the start of every step_epochs call and drops at its end
+ *    (optimizer.rs:314, :323-324);
+ *  UMAPGPU_SEMANTICS_UMAP_LEARN_FORCES: force denominators a*d^(2b) + 1 as
+ *    in umap-learn instead of the reference's a*d^(2b)*d^2 + 1
+ *    (layout/optimize_layout_euclidean.rs:293, :329; Euclidean path only).
+ * The serial mode stays a deterministic, oracle-checked restatement under
+ * every setting. */
+#define UMAPGPU_SEMANTICS_SHARED_BUFFER 1u
+#define UMAPGPU_SEMANTICS_UMAP_LEARN_FORCES 2u
+UMAPGPU_API umapgpu_status umapgpu_optimizer_set_semantics(umapgpu_optimizer* o, uint32_t flags);
+UMAPGPU_API uint32_t umapgpu_optimizer_semantics(const umapgpu_optimizer* o);
 /* current_epoch / total_epochs / remaining_epochs (optimizer.rs:369-381) */
 UMAPGPU_API uint64_t umapgpu_optimizer_current_epoch(const umapgpu_optimizer* opt);
 UMAPGPU_API uint64_t umapgpu_optimizer_total_epochs(const umapgpu_optimizer* opt);

@@ -57,7 +57,9 @@ def lib():
         L.orc_sgd_epoch_serial.argtypes = epoch_args + [P]
         L.orc_sgd_epoch_hogwild.argtypes = epoch_args + [P]
         L.orc_sgd_epoch_generic_euclid.argtypes = epoch_args
-        L.orc_step_epochs.argtypes = [P, u64, u64, P, P, u64, P, P, P, P, f32, f32, f32, f32, u64, u64, u64, u64, i32, P]
+        L.orc_set_force_denominator.argtypes = [i32]
+        L.orc_step_epochs.argtypes = [P, u64, u64, P, P, u64, P, P, P, P, f32, f32, f32, f32, u64, u64, u64, u64, i32, P,
+                                      i32]
         L.orc_num_threads.restype = i32
         _lib = L
     return _lib
@@ -186,7 +188,7 @@ class OptimizerState:
     """Optimizer (optimizer.rs:28-56) state after Optimizer::new (:74-290)."""
 
     def __init__(self, manifold: Manifold, init, total_epochs, negative_sample_rate=5, learning_rate=1.0,
-                 repulsion_strength=1.0, seed=0):
+                 repulsion_strength=1.0, seed=0, semantics=0):
         emb = np.array(init, dtype=np.float32, order="C", copy=True)
         n, dim = emb.shape
         ph, pt, pe, po, pn, pon = (P() for _ in range(6))
@@ -210,6 +212,7 @@ class OptimizerState:
         self.total_epochs = total_epochs
         self.current_epoch = 0
         self.seed = seed
+        self.semantics = semantics  # bit 1: shared array; bit 2: umap-learn force denominators
         self.counts = np.zeros(2, np.uint64)
 
     @property
@@ -224,7 +227,8 @@ class OptimizerState:
         lib().orc_step_epochs(_ptr(self.embedding), self.n, self.dim, _ptr(self.head), _ptr(self.tail),
                               self.n_edges, _ptr(self.eps), _ptr(self.eons), _ptr(self.epns), _ptr(self.eonns),
                               self.a, self.b, self.gamma, self.initial_alpha, self.current_epoch, n_epochs,
-                              self.total_epochs, self.seed, 0 if mode == "serial" else 1, _ptr(self.counts))
+                              self.total_epochs, self.seed, 0 if mode == "serial" else 1, _ptr(self.counts),
+                              self.semantics)
         self.current_epoch += n_epochs
 
     def step_epochs_generic_euclid(self, n_epochs: int):

@@ -517,6 +517,13 @@ static inline uint64_t sat_u64(double q) {
   return (uint64_t)q;
 }
 
+/* Force denominators: the reference's a*d2^b*d2 + 1 (ref_den = 1,
+ * optimize_layout_euclidean.rs:293, :329) or umap-learn's a*d2^b + 1. */
+static int g_ref_den = 1;
+
+/* Test hook for the single-epoch entry points: 1 = reference denominators. */
+ORC_API void orc_set_force_denominator(int32_t reference) { g_ref_den = reference ? 1 : 0; }
+
 /* One edge of the Euclidean SGD epoch: optimize_layout_euclidean.rs:277-344
  * (parallel body; the serial body :168-231 is the same arithmetic).         */
 static inline void sgd_edge(float *head_emb, float *tail_emb, uint64_t dim, uint32_t j, uint32_t k, uint64_t i,
@@ -535,7 +542,7 @@ static inline void sgd_edge(float *head_emb, float *tail_emb, uint64_t dim, uint
   if (d2 > 0.0f) {
     float pb = powf(d2, b);
     gc = (((-2.0f * a) * b) * pb) / d2;
-    gc = gc / ((a * pb) * d2 + 1.0f);
+    gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
   }
   for (uint64_t d = 0; d < dim; d++) {
     float diff = cur[d] - oth[d];
@@ -558,7 +565,7 @@ static inline void sgd_edge(float *head_emb, float *tail_emb, uint64_t dim, uint
     float g2 = 0.0f;
     if (dd > 0.0f) {
       float pb = powf(dd, b);
-      g2 = ((2.0f * gamma) * b) / ((0.001f + dd) * ((a * pb) * dd + 1.0f));
+      g2 = ((2.0f * gamma) * b) / ((0.001f + dd) * ((a * pb) * (g_ref_den ? dd : 1.0f) + 1.0f));
     }
     if (g2 > 0.0f)
       for (uint64_t d = 0; d < dim; d++) {
@@ -608,15 +615,19 @@ ORC_API void orc_sgd_epoch_hogwild(float *head_emb, float *tail_emb, uint64_t di
 }
 
 /* Optimizer::step_epochs (optimizer.rs:297-366): tail snapshot per call,
- * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild.     */
+ * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild.
+ * semantics: bit 1 = one shared array instead of the tail snapshot, bit 2 =
+ * umap-learn's force denominators (both opt-in departures; 0 = reference). */
 ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const uint32_t *head, const uint32_t *tail,
                              uint64_t n_edges, const double *eps, double *eons, const double *epns, double *eonns,
                              float a, float b, float gamma, float initial_alpha, uint64_t start_epoch,
                              uint64_t n_epochs, uint64_t total_epochs, uint64_t seed, int32_t mode,
-                             uint64_t *counts) {
+                             uint64_t *counts, int32_t semantics) {
   uint64_t nel = n * dim;
-  float *copy = (float *)malloc((nel > 0 ? nel : 1) * sizeof(float));
-  memcpy(copy, embedding, n * dim * sizeof(float));
+  const int shared = (semantics & 1) != 0;
+  g_ref_den = (semantics & 2) ? 0 : 1;
+  float *copy = shared ? embedding : (float *)malloc((nel > 0 ? nel : 1) * sizeof(float));
+  if (!shared) memcpy(copy, embedding, n * dim * sizeof(float));
   for (uint64_t e = start_epoch; e < start_epoch + n_epochs; e++) {
     float alpha = initial_alpha * (1.0f - ((float)e / (float)total_epochs));
     if (mode == 0)
@@ -626,7 +637,8 @@ ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const u
       orc_sgd_epoch_hogwild(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
                             gamma, alpha, e, seed, counts);
   }
-  free(copy);
+  if (!shared) free(copy);
+  g_ref_den = 1;
 }
 
 /* Generic-metric epoch: layout/optimize_layout_generic.rs:35-124 with the

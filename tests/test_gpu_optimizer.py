@@ -204,3 +204,48 @@ def test_hogwild_modes_share_schedule(gpu, oracle, n_components):
         assert np.all(np.isfinite(o.embedding()))
         st = o.export_state()
         assert np.array_equal(st["epoch_of_next_sample"], ost.eons)
+
+
+@pytest.mark.parametrize("semantics", [1, 2, 3])
+def test_serial_trajectory_bit_identical_under_opt_in_semantics(gpu, oracle, semantics):
+    """The opt-in semantics (shared array, umap-learn force denominators) keep
+    the serial mode bit-identical to the restatement run with the same bits."""
+    _, _, _, opt, ost = _setup(gpu, oracle, n=800, total=30)
+    ost.semantics = semantics
+    opt.set_mode(gpu.SGD_SERIAL)
+    opt.set_semantics(semantics)
+    assert opt.semantics() == semantics
+    for chunk in (3, 9):
+        opt.step_epochs(chunk)
+        ost.step_epochs(chunk, "serial")
+        assert np.array_equal(opt.embedding().view(np.uint32), ost.embedding.view(np.uint32)), chunk
+    p, q, _ = opt.stats()
+    assert (p, q) == tuple(int(c) for c in ost.counts)
+
+
+def test_opt_in_semantics_restore_cluster_structure(gpu, oracle):
+    """With umap-learn's forces on one shared array the Hogwild fit keeps the
+    mixture's clusters apart; with the reference's semantics it mixes them
+    (the reference's own behaviour, as its CPU restatement shows)."""
+    import umap_eval as E
+    x, lab = gmm(6000, 16, 8, seed=4)
+    idx, d = knn_exact(x, 15)
+    cfg = gpu.UmapConfig()
+    cfg.graph.n_neighbors = 15
+    m = gpu.Umap(cfg).learn_manifold(x, idx, d)
+    init = uniform_init(6000, 2, seed=5)
+    q = np.arange(0, 6000, 7)
+
+    def purity(y):
+        low = E._knn_rows(y, q, 15)
+        return float((lab[low] == lab[q][:, None]).mean())
+
+    res = {}
+    for sem in (0, 3):
+        opt = gpu.Optimizer.new(m.clone(), init, 200, cfg, gpu.MetricType.Euclidean, seed=1)
+        opt.set_semantics(sem)
+        opt.step_epochs(200)
+        res[sem] = purity(opt.embedding())
+    print(f"[semantics] purity reference={res[0]:.3f} umap-learn={res[3]:.3f}", flush=True)
+    assert res[3] > 0.95
+    assert res[3] > res[0] + 0.2

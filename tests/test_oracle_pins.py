@@ -297,6 +297,56 @@ def test_sgd_repulsion_known_answer(oracle):
     assert eonns[0] == 1.0
 
 
+def test_sgd_umap_learn_denominators_known_answer(oracle):
+    """Opt-in umap-learn forces: attraction -2ab d2^(b-1) / (a d2^b + 1) and
+    repulsion 2 gamma b / ((0.001 + d2)(a d2^b + 1)) (umap-learn's layouts.py)."""
+    a, b, gamma, alpha = F(1.2), F(0.95), F(1.3), F(0.5)
+    H = np.array([[0.0, 0.0], [0.3, 0.4], [1.0, 2.0]], F)
+    T = H.copy()
+    head, tail = np.array([0], np.uint32), np.array([1], np.uint32)
+    eps, eons = np.array([1.0]), np.array([1.0])
+    epns, eonns = np.array([1.0]), np.array([0.0])
+    k = oracle.rng_index(0, 1, 0, 0, 3)
+    H0 = H.copy()
+    oracle.lib().orc_set_force_denominator(0)
+    try:
+        _run_epoch(oracle, H, T, head, tail, 3, eps, eons, epns, eonns, a, b, gamma, alpha, 1)
+    finally:
+        oracle.lib().orc_set_force_denominator(1)
+    h, t1 = H0[0].astype(np.float64), H0[1].astype(np.float64)
+    d2 = ((h - t1) ** 2).sum()
+    gc = -2 * float(a) * float(b) * d2 ** (float(b) - 1) / (float(a) * d2 ** float(b) + 1)
+    g = np.clip(gc * (h - t1), -4, 4) * float(alpha)
+    h = h + g
+    t_after = H0.astype(np.float64).copy()
+    t_after[1] -= g
+    if k != 0:
+        o = t_after[k]
+        d2 = ((h - o) ** 2).sum()
+        gr = 2 * float(gamma) * float(b) / ((0.001 + d2) * (float(a) * d2 ** float(b) + 1))
+        h = h + np.clip(gr * (h - o), -4, 4) * float(alpha)
+    assert np.allclose(H[0], h, rtol=1e-5)
+
+
+def test_shared_buffer_semantics_see_every_update(oracle):
+    """With one shared array, the second edge reads its head row after the
+    first edge moved it as a tail; with the snapshot it reads the old value."""
+    from oracle import Manifold, OptimizerState
+    # two edges 0->1 and 1->0 of weight 1 over 2 points
+    m = Manifold(np.array([0, 1, 2], np.uint64), np.array([1, 0], np.uint32), np.array([1.0, 1.0], F),
+                 np.ones(2, F), np.zeros(2, F), 2, F(1.5), F(0.9))
+    init = np.array([[0.0, 0.0], [1.0, 0.0]], F)
+    outs = []
+    for sem in (0, 1):
+        st = OptimizerState(m, init, 10, negative_sample_rate=0, seed=0, semantics=sem)
+        st.step_epochs(2, "serial")
+        outs.append(st.embedding.copy())
+    assert not np.array_equal(outs[0], outs[1])
+    # snapshot semantics: each head row only moves by its own head updates, so
+    # the two points move symmetrically about the midpoint
+    assert outs[0][0, 0] + outs[0][1, 0] == pytest.approx(10.0, abs=1e-5)
+
+
 def test_sgd_gradient_clip(oracle):
     """A repulsive sample at distance 0.1 has |coefficient * diff| ~ 16, which
     clip (utils/clip.rs:13-15) cuts to 4: the head moves by exactly 4 * alpha."""

@@ -357,7 +357,12 @@ void step_generic(OptimizerState& st, uint64_t n_epochs, umapgpu_distance_fn dis
   std::vector<double> eps(E), eons(E), epns(E), eonns(E);
   std::vector<float> H(n * dim);
   optimizer_export(st, head.data(), tail.data(), eps.data(), eons.data(), epns.data(), eonns.data(), H.data(), s);
-  std::vector<float> T = H;  // optimizer.rs:314
+  // optimizer.rs:314 snapshots the tail side; shared-buffer semantics use
+  // the embedding itself.  (The generic path's forces never had the
+  // reference's extra d2, so the force-denominator choice does not apply.)
+  std::vector<float> snapshot;
+  if (!(st.semantics & kSemanticsSharedBuffer)) snapshot = H;
+  float* const Tbase = (st.semantics & kSemanticsSharedBuffer) ? H.data() : snapshot.data();
   std::vector<float> g1(dim), g2(dim);
   const float a = st.a, b = st.b, gamma = st.gamma;
   for (uint64_t epoch = st.current_epoch; epoch < st.current_epoch + n_epochs; ++epoch) {
@@ -368,7 +373,7 @@ void step_generic(OptimizerState& st, uint64_t n_epochs, umapgpu_distance_fn dis
       if (!(eons[i] <= (double)epoch)) continue;
       const uint32_t j = head[i], k = tail[i];
       float* cur = H.data() + (uint64_t)j * dim;
-      float* oth = T.data() + (uint64_t)k * dim;
+      float* oth = Tbase + (uint64_t)k * dim;
       const float d = dist(cur, oth, dim, g1.data(), user);
       (void)dist(oth, cur, dim, g2.data(), user);
       const float w_l = (d > 0.0f) ? 1.0f / (1.0f + a * powf(d, 2.0f * b)) : 1.0f;
@@ -383,7 +388,7 @@ void step_generic(OptimizerState& st, uint64_t n_epochs, umapgpu_distance_fn dis
       for (uint32_t p = 0; p < nn; ++p) {
         const uint32_t kk = rng_index(edk, p, (uint32_t)n);
         if (kk == j) continue;
-        float* o2 = T.data() + (uint64_t)kk * dim;
+        float* o2 = Tbase + (uint64_t)kk * dim;
         const float dd = dist(cur, o2, dim, g1.data(), user);
         const float wl = (dd > 0.0f) ? 1.0f / (1.0f + a * powf(dd, 2.0f * b)) : 1.0f;
         const float gcn = (((gamma * 2.0f) * b) * wl) / (dd + 1e-6f);
@@ -658,6 +663,17 @@ umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* o, int32_t mode) {
     o->st.mode = (SgdMode)mode;
   });
 }
+
+umapgpu_status umapgpu_optimizer_set_semantics(umapgpu_optimizer* o, uint32_t flags) {
+  return guarded([&] {
+    if (!o) throw Error(Err::InvalidArgument, "null handle");
+    if (flags & ~(uint32_t)(UMAPGPU_SEMANTICS_SHARED_BUFFER | UMAPGPU_SEMANTICS_UMAP_LEARN_FORCES))
+      throw Error(Err::InvalidArgument, "unknown SGD semantics flags");
+    o->st.semantics = flags;
+  });
+}
+
+uint32_t umapgpu_optimizer_semantics(const umapgpu_optimizer* o) { return o ? o->st.semantics : 0u; }
 
 uint64_t umapgpu_optimizer_current_epoch(const umapgpu_optimizer* o) { return o->st.current_epoch; }
 uint64_t umapgpu_optimizer_total_epochs(const umapgpu_optimizer* o) { return o->st.total_epochs; }

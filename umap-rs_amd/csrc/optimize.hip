@@ -129,6 +129,7 @@ struct SgdArgs {
   uint64_t epoch;
   double epoch_f64;
   uint64_t epoch_key;
+  uint32_t ref_den;  // 1: the reference's force denominators; 0: umap-learn's
 };
 
 __device__ inline void wave_add_stats(uint64_t npos, uint64_t nneg, unsigned long long* stats) {
@@ -142,15 +143,17 @@ __device__ inline void wave_add_stats(uint64_t npos, uint64_t nneg, unsigned lon
   }
 }
 
-// Attractive coefficient (optimize_layout_euclidean.rs:290-297).
+// Attractive coefficient (optimize_layout_euclidean.rs:290-297).  The
+// reference's denominator is a*d2^b*d2 + 1; umap-learn's is a*d2^b + 1, and
+// multiplying by 1.0f instead of d2 gives it exactly.
 __device__ inline float attract_coeff(const SgdArgs& A, float d2, float pb) {
   float gc = (A.m2ab * pb) / d2;
-  gc = gc / ((A.a * pb) * d2 + 1.0f);
+  gc = gc / ((A.a * pb) * (A.ref_den ? d2 : 1.0f) + 1.0f);
   return gc;
 }
-// Repulsive coefficient (:327-332).
+// Repulsive coefficient (:327-332), same choice of denominator.
 __device__ inline float repel_coeff(const SgdArgs& A, float d2, float pb) {
-  return A.g2b / ((0.001f + d2) * ((A.a * pb) * d2 + 1.0f));
+  return A.g2b / ((0.001f + d2) * ((A.a * pb) * (A.ref_den ? d2 : 1.0f) + 1.0f));
 }
 
 // One lane = one head row.  D = embedding dimension (registers).
@@ -303,9 +306,10 @@ __global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H
 }
 
 // Deterministic serial mode: the reference's edge order on one lane
-// (optimize_layout_euclidean.rs:168-231 executed in order).
+// (optimize_layout_euclidean.rs:168-231 executed in order).  H and T may be
+// the same array (shared-buffer semantics), so they carry no restrict.
 template <typename NextT>
-__global__ void sgd_serial_kernel(float* __restrict__ H, float* __restrict__ T, uint32_t dim,
+__global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim,
                                   const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ tail,
                                   const double2* __restrict__ rate, double2* __restrict__ sched,
                                   NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
@@ -628,6 +632,12 @@ uint64_t list_capacity(uint64_t n_edges) {
   return (ceil_div(ceil_div(blocks, kXcds), kShards / kXcds) + 1) * 256;
 }
 
+// The tail-side array: the per-call snapshot, or the embedding itself under
+// shared-buffer semantics.
+inline float* tail_buffer(OptimizerState* st) {
+  return (st->semantics & kSemanticsSharedBuffer) ? st->head_emb.as<float>() : st->tail_emb.as<float>();
+}
+
 template <typename NextT, bool ATOMIC_TAIL>
 void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const uint32_t slot = (uint32_t)(A.epoch & 1u);
@@ -638,7 +648,7 @@ void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   UMAP_LAUNCH_CHECK();
   const unsigned grid = kShards * kBlocksPerShard;
   float* H = st->head_emb.as<float>();
-  float* T = st->tail_emb.as<float>();
+  float* T = tail_buffer(st);
   const uint32_t* hd = st->head.as<uint32_t>();
   const uint32_t* tl = st->tail.as<uint32_t>();
   const double2* rt = st->rate.as<double2>();
@@ -672,7 +682,7 @@ template <typename NextT, bool ATOMIC>
 void launch_rows(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const unsigned grid = grid_for(st->n, 256);
   float* H = st->head_emb.as<float>();
-  float* T = st->tail_emb.as<float>();
+  float* T = tail_buffer(st);
   const uint64_t* rp = st->row_ptr.as<uint64_t>();
   const uint32_t* tl = st->tail.as<uint32_t>();
   const double2* rt = st->rate.as<double2>();
@@ -704,7 +714,7 @@ template <typename NextT>
 void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   switch (st->mode) {
     case SgdMode::Serial:
-      sgd_serial_kernel<NextT><<<1, 64, 0, s>>>(st->head_emb.as<float>(), st->tail_emb.as<float>(),
+      sgd_serial_kernel<NextT><<<1, 64, 0, s>>>(st->head_emb.as<float>(), tail_buffer(st),
                                                 (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
                                                 st->tail.as<uint32_t>(), st->rate.as<double2>(),
                                                 st->sched.as<double2>(), st->next.as<NextT>(), A,
@@ -919,7 +929,7 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
 
 void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
   const uint64_t start = st->current_epoch, end = st->current_epoch + n_epochs;
-  if (st->n * st->dim)
+  if (st->n * st->dim && !(st->semantics & kSemanticsSharedBuffer))
     UMAP_HIP(hipMemcpyAsync(st->tail_emb.p, st->head_emb.p, st->n * st->dim * 4, hipMemcpyDeviceToDevice, s));
   // active-list counters start at zero; within the call each epoch's list
   // kernel clears the counter of the epoch after it
@@ -935,6 +945,7 @@ void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
     A.m2ab = (-2.0f * st->a) * st->b;
     A.g2b = (2.0f * st->gamma) * st->b;
     A.n = (uint32_t)st->n;
+    A.ref_den = (st->semantics & kSemanticsUmapLearnForces) ? 0u : 1u;
     for (uint64_t e = start; e < end; ++e) {
       A.alpha = st->initial_alpha * (1.0f - ((float)e / (float)st->total_epochs));
       A.epoch = e;

@@ -6,6 +6,15 @@ namespace umapgpu {
 
 enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2, HogwildRows = 3 };
 
+// Opt-in departures from the reference's SGD semantics (default 0 = the
+// reference).  kSemanticsSharedBuffer: head and tail updates share one array,
+// as in umap-learn, instead of a tail snapshot taken per step_epochs call
+// (optimizer.rs:314).  kSemanticsUmapLearnForces: force denominators
+// a*d2^b + 1 as in umap-learn instead of the reference's a*d2^b*d2 + 1
+// (optimize_layout_euclidean.rs:293, :329).
+constexpr uint32_t kSemanticsSharedBuffer = 1;
+constexpr uint32_t kSemanticsUmapLearnForces = 2;
+
 // Device-resident Optimizer (optimizer.rs:28-56).  Edges are kept in the
 // reference's CSR row order, so an edge's head is its row in `row_ptr`.
 struct OptimizerState {
@@ -16,6 +25,7 @@ struct OptimizerState {
   int32_t metric_type = 0;
   uint64_t seed = 0;
   SgdMode mode = SgdMode::Hogwild;
+  uint32_t semantics = 0;  // kSemantics* bits
   bool next16 = true;  // next-epoch counters fit in u16 (total_epochs <= 65535)
 
   DevBuf head_emb;  // f32[n*dim]: Optimizer::embedding

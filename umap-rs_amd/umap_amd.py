@@ -35,6 +35,8 @@ P = C.c_void_p
 OK, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, OVERSTEP, NO_DEVICE, UNSUPPORTED = range(7)
 
 SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY, SGD_HOGWILD_ROWS = 0, 1, 2, 3
+# opt-in SGD semantics bits (0 = the reference)
+SEMANTICS_SHARED_BUFFER, SEMANTICS_UMAP_LEARN_FORCES = 1, 2
 
 
 class UmapPanic(RuntimeError):
@@ -104,6 +106,8 @@ def lib():
             "umapgpu_optimizer_step_epochs": ([P, u64], i32),
             "umapgpu_optimizer_step_epochs_generic": ([P, u64, DISTANCE_FN, P], i32),
             "umapgpu_optimizer_set_mode": ([P, i32], i32),
+            "umapgpu_optimizer_set_semantics": ([P, u32], i32),
+            "umapgpu_optimizer_semantics": ([P], u32),
             "umapgpu_optimizer_current_epoch": ([P], u64),
             "umapgpu_optimizer_total_epochs": ([P], u64),
             "umapgpu_optimizer_remaining_epochs": ([P], u64),
@@ -577,6 +581,13 @@ class Optimizer(_Handle):
     def manifold(self) -> LearnedManifold:
         return LearnedManifold._borrowed(lib().umapgpu_optimizer_manifold(self._handle()), self)
 
+    def set_semantics(self, flags: int) -> None:
+        """Opt-in SGD semantics: SEMANTICS_SHARED_BUFFER | SEMANTICS_UMAP_LEARN_FORCES (0 = the reference)."""
+        _check(lib().umapgpu_optimizer_set_semantics(self._handle(), int(flags)))
+
+    def semantics(self) -> int:
+        return int(lib().umapgpu_optimizer_semantics(self._handle()))
+
     def stats(self):
         p, n, ms = u64(), u64(), f64()
         _check(lib().umapgpu_optimizer_stats(self._handle(), C.byref(p), C.byref(n), C.byref(ms)))
@@ -947,6 +958,7 @@ class LearnedInitConfig:
     learning_rate: float = 1e-3
     seed: int = 0
     subsample_init: str = "pca"  # init of the subsample fit: "pca" or "random" (uniform [-10, 10])
+    semantics: int = 0  # SGD semantics bits of the subsample fit (0 = the reference)
 
 
 def learned_init(config: "UmapConfig", x: DeviceArray, li: Optional[LearnedInitConfig] = None):
@@ -989,6 +1001,7 @@ def learned_init(config: "UmapConfig", x: DeviceArray, li: Optional[LearnedInitC
         raise ValueError(f"unknown subsample_init {li.subsample_init!r}")
     opt = Optimizer.new_device(man, init_dev.ptr, m, dim, li.subsample_epochs, cfg, MetricType.Euclidean,
                                seed=li.seed + 1)
+    opt.set_semantics(li.semantics)
     init_dev.free()
     opt.step_epochs(li.subsample_epochs)
     target = opt.embedding_device()
