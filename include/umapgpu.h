@@ -272,6 +272,42 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);
+/* ---- batched sweep (no reference counterpart: the reference steps one
+ * Optimizer at a time, optimizer.rs:297).  Many independent Euclidean fits
+ * (hyperparameter sweeps, per-cluster sub-embeddings) step together: their
+ * edges and embeddings are concatenated so that one persistent edge-parallel
+ * launch per epoch serves every fit, and each call's epoch loop is captured
+ * into one hipGraph.  Every fit keeps its own a, b, learning-rate schedule,
+ * seed and negative-sample range, so each draws exactly the negatives it
+ * would draw alone (the SERIAL mode of a sweep equals each member's own
+ * SERIAL run bit for bit).  The SGD grid (edges per workgroup) is picked by
+ * timing candidate grids on the first epochs of the first Hogwild call.
+ * Members stay owned by the caller and must outlive the sweep; they must
+ * share n_components, the current epoch and the buffer semantics, and must
+ * not be stepped outside the sweep while it exists.  Each sweep step writes
+ * every member's embedding and schedule back and advances its epoch (the
+ * members' own update counters are not advanced; the sweep counts). ---- */
+typedef struct umapgpu_sweep umapgpu_sweep;
+UMAPGPU_API umapgpu_status umapgpu_sweep_create(umapgpu_optimizer* const* members, uint64_t count,
+                                                umapgpu_sweep** out);
+/* Steps every member by n epochs; UMAPGPU_OVERSTEP (reference text) if any
+ * member would pass its total_epochs. */
+UMAPGPU_API umapgpu_status umapgpu_sweep_step_epochs(umapgpu_sweep* s, uint64_t n_epochs);
+/* UMAPGPU_SGD_HOGWILD (default) or UMAPGPU_SGD_SERIAL. */
+UMAPGPU_API umapgpu_status umapgpu_sweep_set_mode(umapgpu_sweep* s, int32_t mode);
+/* 1 (default): capture each call's epoch loop into a hipGraph; 0: launch eagerly. */
+UMAPGPU_API umapgpu_status umapgpu_sweep_set_graph(umapgpu_sweep* s, int32_t use_graph);
+UMAPGPU_API umapgpu_status umapgpu_sweep_info(const umapgpu_sweep* s, uint64_t* fits, uint64_t* rows, uint64_t* edges,
+                                              uint64_t* current_epoch);
+/* Positive updates and negative samples over all calls; the last call's
+ * device time; the chosen SGD blocks per region (grid = 128 x that). */
+UMAPGPU_API umapgpu_status umapgpu_sweep_stats(const umapgpu_sweep* s, uint64_t* positive, uint64_t* negative,
+                                               double* last_step_ms, uint32_t* blocks_per_region);
+/* Autotuner record as JSON ([{"blocks_per_region": b, "ms": t}, ...]); same
+ * length convention as umapgpu_manifold_timings_json. */
+UMAPGPU_API uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t len);
+UMAPGPU_API void umapgpu_sweep_destroy(umapgpu_sweep* s);
+
 /* ---- learned initialisation (no reference counterpart: the reference's fits
  * start from the caller's init, embedding.rs:274-298).  A D -> 256 -> 256 ->
  * out_dim ReLU MLP trained on the embedding a plain fit produced for a

@@ -1,0 +1,114 @@
+"""Batched sweep: many independent fits stepped together on the device.
+
+* SERIAL: a sweep's serial epoch runs each member's edges in order with the
+  member's own parameters, RNG keys and negative-sample range, so every
+  member ends bit-identical to its own serial run.
+* HOGWILD: the RNG-free schedule makes the update counts equal the sum over
+  members run alone; each member's fuzzy cross-entropy lands within 5% of its
+  own Hogwild run.  The hipGraph-captured loop is compared with eager launches
+  over several calls.
+"""
+import numpy as np
+import pytest
+
+import umap_eval as E
+from helpers import gmm, knn_exact, uniform_init
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [  # (n, dim of data, k, min_dist, seed, semantics)
+    (700, 12, 10, 0.1, 1, 0),
+    (1500, 16, 15, 0.5, 2, 0),
+    (400, 8, 10, 0.0, 3, 2),
+    (1100, 20, 15, 0.1, 4, 2),
+]
+
+
+def _manifold(U, n, dim, k, min_dist, seed):
+    x, _ = gmm(n, dim, 5, seed=seed)
+    idx, d = knn_exact(x, k)
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = k
+    cfg.manifold.min_dist = min_dist
+    return U.Umap(cfg).learn_manifold(x, idx, d), cfg
+
+
+def _members(U, total, specs=SPECS):
+    """Two identical optimizers per spec: one for the sweep, one run alone."""
+    a, b = [], []
+    for n, dim, k, md, seed, sem in specs:
+        m, cfg = _manifold(U, n, dim, k, md, seed)
+        init = uniform_init(n, 2, seed=seed + 50)
+        for out in (a, b):
+            o = U.Optimizer.new(m.clone(), init, total + seed, cfg, U.MetricType.Euclidean, seed=seed)
+            o.set_semantics(sem)
+            out.append(o)
+    return a, b
+
+
+def test_sweep_serial_equals_members_alone(gpu):
+    U = gpu
+    swept, alone = _members(U, 30)
+    sw = U.Sweep(swept)
+    sw.set_mode(U.SGD_SERIAL)
+    info = sw.info()
+    assert info["fits"] == len(SPECS) and info["current_epoch"] == 0
+    for chunk in (1, 6):
+        sw.step_epochs(chunk)
+        for o in alone:
+            o.set_mode(U.SGD_SERIAL)
+            o.step_epochs(chunk)
+        for i, (s_opt, a_opt) in enumerate(zip(swept, alone)):
+            assert s_opt.current_epoch() == a_opt.current_epoch()
+            assert np.array_equal(s_opt.embedding().view(np.uint32), a_opt.embedding().view(np.uint32)), (i, chunk)
+            ss, sa = s_opt.export_state(), a_opt.export_state()
+            assert np.array_equal(ss["epoch_of_next_sample"], sa["epoch_of_next_sample"]), i
+            assert np.array_equal(ss["epoch_of_next_negative_sample"], sa["epoch_of_next_negative_sample"]), i
+    st = sw.stats()
+    assert st["positive"] == sum(o.stats()[0] for o in alone)
+    assert st["negative"] == sum(o.stats()[1] for o in alone)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_sweep_hogwild_counts_and_quality(gpu, use_graph):
+    U = gpu
+    total = 60
+    swept, alone = _members(U, total)
+    sw = U.Sweep(swept)
+    sw.set_graph(use_graph)
+    for chunk in (20, 15, 25):  # several calls: each captures and replays its own graph
+        sw.step_epochs(chunk)
+        for o in alone:
+            o.step_epochs(chunk)
+    st = sw.stats()
+    assert st["positive"] == sum(o.stats()[0] for o in alone)
+    assert st["negative"] == sum(o.stats()[1] for o in alone)
+    tun = sw.tuning()
+    assert len(tun) == 4 and st["blocks_per_region"] in [t["blocks_per_region"] for t in tun]
+    for i, (s_opt, a_opt) in enumerate(zip(swept, alone)):
+        assert s_opt.current_epoch() == 60
+        ss, sa = s_opt.export_state(), a_opt.export_state()
+        assert np.array_equal(ss["epoch_of_next_sample"], sa["epoch_of_next_sample"]), i
+        m = a_opt.manifold()
+        g = m.graph()
+        a, b = m.curve_params()
+        ce_s = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), s_opt.embedding(), a, b)
+        ce_a = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), a_opt.embedding(), a, b)
+        print(f"[sweep] fit {i} graph={use_graph} CE sweep={ce_s:.5g} alone={ce_a:.5g}", flush=True)
+        assert abs(ce_s - ce_a) / ce_a < 0.05, i
+
+
+def test_sweep_overstep_and_validation(gpu):
+    U = gpu
+    swept, _ = _members(U, 10, SPECS[:2])
+    sw = U.Sweep(swept)
+    with pytest.raises(U.UmapPanic, match="Cannot step 12 epochs: would exceed total_epochs 11"):
+        sw.step_epochs(12)
+    sw.step_epochs(3)
+    assert [o.current_epoch() for o in swept] == [3, 3]
+    # members must share the epoch
+    third, _ = _members(U, 10, SPECS[2:3])
+    with pytest.raises(U.UmapPanic, match="share the current epoch"):
+        U.Sweep([swept[0], third[0]])
+    with pytest.raises(U.UmapPanic, match="twice"):
+        U.Sweep([third[0], third[0]])

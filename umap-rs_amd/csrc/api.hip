@@ -230,6 +230,10 @@ struct umapgpu_mlp {
   MlpState st;
 };
 
+struct umapgpu_sweep {
+  SweepState sw;
+};
+
 struct umapgpu_fitted {
   DevBuf embedding;
   uint64_t n = 0, dim = 0;
@@ -891,6 +895,90 @@ umapgpu_status umapgpu_knn_bruteforce_device(const float* data, uint64_t n, uint
     knn_bruteforce(data, n, dim, k, idx, dists, stream());
   });
 }
+
+umapgpu_status umapgpu_sweep_create(umapgpu_optimizer* const* members, uint64_t count, umapgpu_sweep** out) {
+  return guarded([&] {
+    ensure_device();
+    if (!members || !out) throw Error(Err::InvalidArgument, "null argument");
+    std::vector<OptimizerState*> ms;
+    for (uint64_t i = 0; i < count; ++i) {
+      if (!members[i]) throw Error(Err::InvalidArgument, "null optimizer in sweep");
+      for (OptimizerState* q : ms)
+        if (q == &members[i]->st) throw Error(Err::InvalidArgument, "an optimizer appears twice in the sweep");
+      ms.push_back(&members[i]->st);
+    }
+    auto sw = std::make_unique<umapgpu_sweep>();
+    sweep_build(&sw->sw, ms, stream());
+    *out = sw.release();
+  });
+}
+
+umapgpu_status umapgpu_sweep_step_epochs(umapgpu_sweep* s, uint64_t n) {
+  return guarded([&] {
+    ensure_device();
+    if (!s) throw Error(Err::InvalidArgument, "null handle");
+    sweep_step(&s->sw, n, stream());
+  });
+}
+
+umapgpu_status umapgpu_sweep_set_mode(umapgpu_sweep* s, int32_t mode) {
+  return guarded([&] {
+    if (!s) throw Error(Err::InvalidArgument, "null handle");
+    if (mode != UMAPGPU_SGD_HOGWILD && mode != UMAPGPU_SGD_SERIAL)
+      throw Error(Err::InvalidArgument, "a sweep runs the Hogwild or the serial mode");
+    s->sw.mode = (SgdMode)mode;
+  });
+}
+
+umapgpu_status umapgpu_sweep_set_graph(umapgpu_sweep* s, int32_t use_graph) {
+  return guarded([&] {
+    if (!s) throw Error(Err::InvalidArgument, "null handle");
+    s->sw.use_graph = use_graph != 0;
+  });
+}
+
+umapgpu_status umapgpu_sweep_info(const umapgpu_sweep* s, uint64_t* fits, uint64_t* rows, uint64_t* edges,
+                                  uint64_t* cur) {
+  return guarded([&] {
+    if (!s) throw Error(Err::InvalidArgument, "null handle");
+    if (fits) *fits = s->sw.members.size();
+    if (rows) *rows = s->sw.total_n;
+    if (edges) *edges = s->sw.total_e;
+    if (cur) *cur = s->sw.current_epoch;
+  });
+}
+
+umapgpu_status umapgpu_sweep_stats(const umapgpu_sweep* s, uint64_t* pos, uint64_t* neg, double* ms, uint32_t* bps) {
+  return guarded([&] {
+    ensure_device();
+    if (!s) throw Error(Err::InvalidArgument, "null handle");
+    uint64_t p = 0, q = 0;
+    sweep_read_stats(s->sw, &p, &q, stream());
+    if (pos) *pos = p;
+    if (neg) *neg = q;
+    if (ms) *ms = s->sw.last_step_ms;
+    if (bps) *bps = s->sw.blocks_per_shard;
+  });
+}
+
+uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t len) {
+  if (!s) return 0;
+  std::string j = "[";
+  for (size_t i = 0; i < s->sw.tuning.size(); ++i) {
+    if (i) j += ",";
+    j += "{\"blocks_per_region\":" + std::to_string(s->sw.tuning[i].first) +
+         ",\"ms\":" + std::to_string(s->sw.tuning[i].second) + "}";
+  }
+  j += "]";
+  if (buf && len) {
+    const size_t c = std::min<size_t>(j.size(), len - 1);
+    memcpy(buf, j.data(), c);
+    buf[c] = 0;
+  }
+  return j.size() + 1;
+}
+
+void umapgpu_sweep_destroy(umapgpu_sweep* s) { delete s; }
 
 void umapgpu_mlp_config_default(umapgpu_mlp_config* c) {
   if (!c) return;

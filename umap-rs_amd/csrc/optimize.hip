@@ -132,6 +132,13 @@ struct SgdArgs {
   uint32_t ref_den;  // 1: the reference's force denominators; 0: umap-learn's
 };
 
+// Per-lane view of the parameters an edge's update needs.
+struct LaneFit {
+  float a, b, m2ab, g2b, alpha;
+  uint32_t n, ref_den, row_off;
+  uint64_t epoch_key;
+};
+
 __device__ inline void wave_add_stats(uint64_t npos, uint64_t nneg, unsigned long long* stats) {
   for (int off = 32; off > 0; off >>= 1) {
     npos += __shfl_xor(npos, off, 64);
@@ -145,14 +152,17 @@ __device__ inline void wave_add_stats(uint64_t npos, uint64_t nneg, unsigned lon
 
 // Attractive coefficient (optimize_layout_euclidean.rs:290-297).  The
 // reference's denominator is a*d2^b*d2 + 1; umap-learn's is a*d2^b + 1, and
-// multiplying by 1.0f instead of d2 gives it exactly.
-__device__ inline float attract_coeff(const SgdArgs& A, float d2, float pb) {
+// multiplying by 1.0f instead of d2 gives it exactly.  P is SgdArgs or a
+// sweep fit's parameters (same field names).
+template <typename P>
+__device__ inline float attract_coeff(const P& A, float d2, float pb) {
   float gc = (A.m2ab * pb) / d2;
   gc = gc / ((A.a * pb) * (A.ref_den ? d2 : 1.0f) + 1.0f);
   return gc;
 }
 // Repulsive coefficient (:327-332), same choice of denominator.
-__device__ inline float repel_coeff(const SgdArgs& A, float d2, float pb) {
+template <typename P>
+__device__ inline float repel_coeff(const P& A, float d2, float pb) {
   return A.g2b / ((0.001f + d2) * ((A.a * pb) * (A.ref_den ? d2 : 1.0f) + 1.0f));
 }
 
@@ -307,15 +317,29 @@ __global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H
 
 // Deterministic serial mode: the reference's edge order on one lane
 // (optimize_layout_euclidean.rs:168-231 executed in order).  H and T may be
-// the same array (shared-buffer semantics), so they carry no restrict.
-template <typename NextT>
-__global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim,
-                                  const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ tail,
-                                  const double2* __restrict__ rate, double2* __restrict__ sched,
-                                  NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
+// the same array (shared-buffer semantics), so they carry no restrict.  A
+// sweep runs its fits one after another, each with its own parameters.
+template <typename NextT, bool SWEEP>
+__global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64_t* __restrict__ row_ptr,
+                                  const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
+                                  double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A,
+                                  const SweepFit* __restrict__ fits, const SweepEpoch* __restrict__ fit_epoch,
+                                  uint32_t nfits, unsigned long long* __restrict__ stats) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   uint64_t npos = 0, nneg = 0;
+  uint32_t f = 0;
   for (uint32_t j = 0; j < A.n; ++j) {
+    LaneFit F;
+    uint64_t edge_off = 0;
+    if constexpr (SWEEP) {
+      while (f + 1 < nfits && j >= fits[f + 1].row_off) ++f;
+      const SweepFit fp = fits[f];
+      F = LaneFit{fp.a, fp.b, fp.m2ab, fp.g2b, fit_epoch[f].alpha, fp.n, fp.ref_den, fp.row_off,
+                  fit_epoch[f].epoch_key};
+      edge_off = fp.edge_off;
+    } else {
+      F = LaneFit{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
+    }
     float* h = H + (uint64_t)j * dim;
     for (uint64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
       double2 sc = sched[e];
@@ -328,19 +352,19 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim,
         d2 = d2 + df * df;
       }
       float gc = 0.0f;
-      if (d2 > 0.0f) gc = attract_coeff(A, d2, libm_powf(d2, A.b));
+      if (d2 > 0.0f) gc = attract_coeff(F, d2, libm_powf(d2, F.b));
       for (uint32_t d = 0; d < dim; ++d) {
         const float df = h[d] - t[d];
         const float g = clip4(gc * df);
-        h[d] = h[d] + g * A.alpha;
-        t[d] = t[d] - g * A.alpha;
+        h[d] = h[d] + g * F.alpha;
+        t[d] = t[d] - g * F.alpha;
       }
       const double2 rt = rate[e];
       sc.x = sc.x + rt.x;
       const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);
-      const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+      const uint64_t ekey = rng_edge_key(F.epoch_key, e - edge_off);
       for (uint32_t p = 0; p < nn; ++p) {
-        const uint32_t kk = rng_index(ekey, p, A.n);
+        const uint32_t kk = F.row_off + rng_index(ekey, p, F.n);
         if (kk == j) continue;
         const float* o = T + (uint64_t)kk * dim;
         float dd = 0.0f;
@@ -349,11 +373,11 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim,
           dd = dd + df * df;
         }
         float g2 = 0.0f;
-        if (dd > 0.0f) g2 = repel_coeff(A, dd, libm_powf(dd, A.b));
+        if (dd > 0.0f) g2 = repel_coeff(F, dd, libm_powf(dd, F.b));
         if (g2 > 0.0f)
           for (uint32_t d = 0; d < dim; ++d) {
             const float df = h[d] - o[d];
-            h[d] = h[d] + clip4(g2 * df) * A.alpha;
+            h[d] = h[d] + clip4(g2 * df) * F.alpha;
           }
       }
       sc.y = sc.y + (double)nn * rt.y;
@@ -454,21 +478,34 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
 // the first kPrefetch negative targets, whose indices depend only on the
 // counter RNG -- is issued before any of them is used, so the lane waits
 // for one memory round trip instead of one per sample.
-template <int D, typename NextT, bool ATOMIC_TAIL>
+//
+// SWEEP: the edge array is the concatenation of independent fits (a batched
+// sweep).  Each lane finds its edge's fit by binary search over the fits'
+// edge offsets in LDS and takes that fit's a, b, alpha, vertex count, row
+// offset and RNG key; negative samples stay inside the fit's own rows and
+// use the fit-local edge index, so every fit draws what it would alone.
+template <int D, typename NextT, bool ATOMIC_TAIL, bool SWEEP>
 __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
                                                         const uint32_t* __restrict__ head,
                                                         const uint32_t* __restrict__ tail,
                                                         const double2* __restrict__ rate, double2* __restrict__ sched,
                                                         NextT* __restrict__ next, const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ counters, uint32_t slot,
-                                                        uint64_t cap, SgdArgs A, unsigned long long* __restrict__ stats) {
+                                                        uint64_t cap, SgdArgs A, const SweepFit* __restrict__ fits,
+                                                        const SweepEpoch* __restrict__ fit_epoch, uint32_t nfits,
+                                                        unsigned long long* __restrict__ stats) {
   // powf's two lookup tables, staged in LDS for low-latency divergent reads
   __shared__ uint64_t s_exp2[32];
   __shared__ double s_log2[32];
+  __shared__ uint64_t s_eoff[SWEEP ? kMaxSweepFits + 1 : 1];
   if (UMAP_SGD_LDS_TABLES && threadIdx.x < 32) {
     const LibmTables c = libm_default_tables();
     s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
     s_log2[threadIdx.x] = c.log2[threadIdx.x];
+  }
+  if constexpr (SWEEP) {
+    for (uint32_t f = threadIdx.x; f <= nfits; f += blockDim.x)
+      s_eoff[f] = f < nfits ? fits[f].edge_off : ~0ull;
   }
   __syncthreads();
   const LibmTables tb = UMAP_SGD_LDS_TABLES ? LibmTables{s_exp2, s_log2} : libm_default_tables();
@@ -492,6 +529,22 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
       const uint32_t e = __builtin_nontemporal_load(lst + i);
       j = __builtin_nontemporal_load(head + e);
       const uint32_t k = __builtin_nontemporal_load(tail + e);
+      // this edge's fit: the whole problem, or one member of a sweep
+      LaneFit F;
+      uint64_t e_local = e;
+      if constexpr (SWEEP) {
+        uint32_t lo = 0, hi = nfits;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_eoff[mid] <= (uint64_t)e) lo = mid; else hi = mid;
+        }
+        const SweepFit fp = fits[lo];
+        const SweepEpoch fe = fit_epoch[lo];
+        F = LaneFit{fp.a, fp.b, fp.m2ab, fp.g2b, fe.alpha, fp.n, fp.ref_den, fp.row_off, fe.epoch_key};
+        e_local = e - fp.edge_off;
+      } else {
+        F = LaneFit{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
+      }
       double2 sc, rt;
       sc.x = __builtin_nontemporal_load(&sched[e].x);
       sc.y = __builtin_nontemporal_load(&sched[e].y);
@@ -503,14 +556,14 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
       __builtin_nontemporal_store(sc.x, &sched[e].x);
       __builtin_nontemporal_store(sc.y, &sched[e].y);
       __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
-      const uint64_t ekey = rng_edge_key(A.epoch_key, e);
+      const uint64_t ekey = rng_edge_key(F.epoch_key, e_local);
       // gathers, all in flight together; a draw equal to j is skipped later
       // (the draw still counts, :315-317)
       uint32_t self_mask = 0;
       float tn[kPrefetch][D];
 #pragma unroll
       for (int q = 0; q < kPrefetch; ++q) {
-        const uint32_t kq = rng_index(ekey, (uint64_t)q, A.n);
+        const uint32_t kq = F.row_off + rng_index(ekey, (uint64_t)q, F.n);
         self_mask |= (kq == j) ? (1u << q) : 0u;
         if ((uint32_t)q < nn) load_trow<D>(T, trsrc, kq, tn[q]);
       }
@@ -527,11 +580,11 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
           diff[d] = h[d] - tv[d];
           d2 = d2 + diff[d] * diff[d];
         }
-        const float gc = (d2 > 0.0f) ? attract_coeff(A, d2, libm_powf(d2, A.b, tb)) : 0.0f;
+        const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
         float ga[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          ga[d] = clip4(gc * diff[d]) * A.alpha;
+          ga[d] = clip4(gc * diff[d]) * F.alpha;
           h[d] = h[d] + ga[d];
         }
         if (ATOMIC_TAIL && UMAP_SGD_TAIL_CAS && D == 2) {
@@ -571,7 +624,7 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
               for (int d = 0; d < D; ++d) o[d] = tn[q][d];
             }
         } else {
-          const uint32_t kp = rng_index(ekey, p, A.n);
+          const uint32_t kp = F.row_off + rng_index(ekey, p, F.n);
           skip = (kp == j);
           load_trow<D>(T, trsrc, kp, o);
         }
@@ -582,10 +635,10 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
           diff[d] = h[d] - o[d];
           d2 = d2 + diff[d] * diff[d];
         }
-        const float g2 = (d2 > 0.0f) ? repel_coeff(A, d2, libm_powf(d2, A.b, tb)) : 0.0f;
+        const float g2 = (d2 > 0.0f) ? repel_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
         if (g2 > 0.0f) {
 #pragma unroll
-          for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * A.alpha;
+          for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * F.alpha;
         }
       }
 #pragma unroll
@@ -617,6 +670,24 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
     }
   }
   wave_add_stats(npos, nneg, stats);
+}
+
+__global__ void add_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n,
+                               uint32_t add) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] + add;
+}
+
+__global__ void add_u64_kernel(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src, uint64_t n,
+                               uint64_t add) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] + add;
+}
+
+// Plain copy as a kernel, so that it can sit inside a captured graph.
+__global__ void copy_f32_kernel(float* __restrict__ dst, const float* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void fill_head_kernel(const uint64_t* __restrict__ row_ptr, uint64_t n, uint32_t* __restrict__ head) {
@@ -659,8 +730,8 @@ void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   unsigned long long* stt = st->stats.as<unsigned long long>();
 #define UMAP_EDGES_CASE(DD)                                                                                       \
   case DD:                                                                                                        \
-    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot, cap, A, \
-                                                                  stt);                                           \
+    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL, false><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot,   \
+                                                                         cap, A, nullptr, nullptr, 0u, stt);      \
     break;
   switch (st->dim) {
     UMAP_EDGES_CASE(1)
@@ -714,11 +785,11 @@ template <typename NextT>
 void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   switch (st->mode) {
     case SgdMode::Serial:
-      sgd_serial_kernel<NextT><<<1, 64, 0, s>>>(st->head_emb.as<float>(), tail_buffer(st),
-                                                (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
-                                                st->tail.as<uint32_t>(), st->rate.as<double2>(),
-                                                st->sched.as<double2>(), st->next.as<NextT>(), A,
-                                                st->stats.as<unsigned long long>());
+      sgd_serial_kernel<NextT, false><<<1, 64, 0, s>>>(st->head_emb.as<float>(), tail_buffer(st),
+                                                       (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
+                                                       st->tail.as<uint32_t>(), st->rate.as<double2>(),
+                                                       st->sched.as<double2>(), st->next.as<NextT>(), A, nullptr,
+                                                       nullptr, 0u, st->stats.as<unsigned long long>());
       UMAP_LAUNCH_CHECK();
       break;
     case SgdMode::HogwildRows:
@@ -1001,6 +1072,277 @@ void optimizer_read_stats(const OptimizerState& st, uint64_t* pos, uint64_t* neg
   UMAP_HIP(hipStreamSynchronize(s));
   if (pos) *pos = v[0];
   if (neg) *neg = v[1];
+}
+
+// ---------------------------------------------------------------------------
+// batched sweep
+
+namespace {
+
+unsigned grid_stride_blocks(uint64_t n) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), 8192)); }
+
+template <typename NextT>
+void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t bps, hipStream_t s) {
+  SgdArgs A{};
+  A.n = (uint32_t)sw->total_n;
+  A.epoch = epoch;
+  A.epoch_f64 = (double)epoch;
+  const uint32_t nf = (uint32_t)sw->members.size();
+  float* H = sw->head_emb.as<float>();
+  float* T = (sw->semantics & kSemanticsSharedBuffer) ? H : sw->tail_emb.as<float>();
+  unsigned long long* stt = sw->stats.as<unsigned long long>();
+  if (sw->mode == SgdMode::Serial) {
+    sgd_serial_kernel<NextT, true><<<1, 64, 0, s>>>(H, T, (uint32_t)sw->dim, sw->row_ptr.as<uint64_t>(),
+                                                     sw->tail.as<uint32_t>(), sw->rate.as<double2>(),
+                                                     sw->sched.as<double2>(), sw->next.as<NextT>(), A,
+                                                     sw->fits.as<SweepFit>(), fe, nf, stt);
+    UMAP_LAUNCH_CHECK();
+    return;
+  }
+  const uint32_t slot = (uint32_t)(epoch & 1u);
+  const uint64_t cap = list_capacity(sw->total_e);
+  active_edges_kernel<NextT><<<grid_for(sw->total_e, 256), 256, 0, s>>>(
+      sw->next.as<NextT>(), sw->total_e, (NextT)epoch, sw->active.as<uint32_t>(), sw->counters.as<uint32_t>(), slot,
+      cap);
+  UMAP_LAUNCH_CHECK();
+  const unsigned grid = kShards * bps;
+#define UMAP_SWEEP_CASE(DD)                                                                                       \
+  case DD:                                                                                                        \
+    sgd_edges_kernel<DD, NextT, true, true><<<grid, 256, 0, s>>>(                                                 \
+        H, T, sw->head.as<uint32_t>(), sw->tail.as<uint32_t>(), sw->rate.as<double2>(), sw->sched.as<double2>(), \
+        sw->next.as<NextT>(), sw->active.as<uint32_t>(), sw->counters.as<uint32_t>(), slot, cap, A,            \
+        sw->fits.as<SweepFit>(), fe, nf, stt);                                                                    \
+    break;
+  switch (sw->dim) {
+    UMAP_SWEEP_CASE(1)
+    UMAP_SWEEP_CASE(2)
+    UMAP_SWEEP_CASE(3)
+    UMAP_SWEEP_CASE(4)
+    UMAP_SWEEP_CASE(5)
+    UMAP_SWEEP_CASE(6)
+    UMAP_SWEEP_CASE(7)
+    UMAP_SWEEP_CASE(8)
+    default:
+      throw Error(Err::Unsupported, "a Hogwild sweep supports n_components <= 8");
+  }
+#undef UMAP_SWEEP_CASE
+  UMAP_LAUNCH_CHECK();
+}
+
+// Candidate SGD grids of the autotuner: 128 regions x blocks per region.
+constexpr uint32_t kTuneBps[] = {4, 8, 16, 32};
+
+template <typename NextT>
+void sweep_run(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
+  const uint32_t nf = (uint32_t)sw->members.size();
+  const uint64_t e0 = sw->current_epoch;
+  // tail snapshot (optimizer.rs:314) and counters, as kernels
+  if (!(sw->semantics & kSemanticsSharedBuffer) && sw->total_n)
+    copy_f32_kernel<<<grid_stride_blocks(sw->total_n * sw->dim), 256, 0, s>>>(
+        sw->tail_emb.as<float>(), sw->head_emb.as<float>(), sw->total_n * sw->dim);
+  fill_u32(sw->counters.as<uint32_t>(), sw->counters.bytes / 4, 0u, s);
+  uint64_t done = 0;
+  const SweepEpoch* fe = sw->fit_epochs.as<SweepEpoch>();
+  // autotune: the first epochs of the first Hogwild call run one candidate
+  // grid each (every candidate computes a valid epoch), timed on the stream
+  if (!sw->tuned && sw->mode != SgdMode::Serial && n_epochs >= sizeof(kTuneBps) / sizeof(kTuneBps[0]) + 1) {
+    hipEvent_t a, b;
+    UMAP_HIP(hipEventCreate(&a));
+    UMAP_HIP(hipEventCreate(&b));
+    float best = 0.0f;
+    for (uint32_t bps : kTuneBps) {
+      UMAP_HIP(hipEventRecord(a, s));
+      sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, bps, s);
+      UMAP_HIP(hipEventRecord(b, s));
+      UMAP_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      UMAP_HIP(hipEventElapsedTime(&ms, a, b));
+      sw->tuning.push_back({bps, ms});
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        sw->blocks_per_shard = bps;
+      }
+      ++done;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    sw->tuned = true;
+  }
+  if (done == n_epochs) return;
+  if (!sw->use_graph) {
+    for (; done < n_epochs; ++done) sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, sw->blocks_per_shard, s);
+    return;
+  }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  UMAP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    for (uint64_t e = done; e < n_epochs; ++e) sweep_epoch<NextT>(sw, e0 + e, fe + e * nf, sw->blocks_per_shard, s);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  UMAP_HIP(hipStreamEndCapture(s, &graph));
+  size_t nodes = 0;
+  (void)hipGraphGetNodes(graph, nullptr, &nodes);
+  sw->graph_nodes = nodes;
+  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    throw_hip(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  hipError_t le = hipGraphLaunch(exec, s);
+  hipError_t se = hipStreamSynchronize(s);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (le != hipSuccess) throw_hip(le, "hipGraphLaunch", __FILE__, __LINE__);
+  if (se != hipSuccess) throw_hip(se, "hipStreamSynchronize", __FILE__, __LINE__);
+}
+
+}  // namespace
+
+void sweep_build(SweepState* sw, const std::vector<OptimizerState*>& members, hipStream_t s) {
+  if (members.empty()) throw Error(Err::InvalidArgument, "a sweep needs at least one optimizer");
+  if (members.size() > kMaxSweepFits) throw Error(Err::Unsupported, "a sweep holds at most 1024 optimizers");
+  const OptimizerState& m0 = *members[0];
+  uint64_t tn = 0, te = 0;
+  for (const OptimizerState* m : members) {
+    if (m->dim != m0.dim) throw Error(Err::InvalidArgument, "sweep members must share n_components");
+    if (m->current_epoch != m0.current_epoch) throw Error(Err::InvalidArgument, "sweep members must share the current epoch");
+    if (m->next16 != m0.next16) throw Error(Err::Unsupported, "sweep members must all have total_epochs <= 65535 or all above");
+    if ((m->semantics & kSemanticsSharedBuffer) != (m0.semantics & kSemanticsSharedBuffer))
+      throw Error(Err::InvalidArgument, "sweep members must share the buffer semantics");
+    if (m->metric_type != 0) throw Error(Err::Unsupported, "a sweep runs the Euclidean path only");
+    tn += m->n;
+    te += m->n_edges;
+  }
+  if (tn >= 0xffffffffull || te >= 0xffffffffull) throw Error(Err::Unsupported, "a sweep holds fewer than 2^32 rows and edges");
+  sw->members = members;
+  sw->dim = m0.dim;
+  sw->total_n = tn;
+  sw->total_e = te;
+  sw->current_epoch = m0.current_epoch;
+  sw->semantics = m0.semantics & kSemanticsSharedBuffer;
+  sw->next16 = m0.next16;
+  const uint64_t D = sw->dim, nb = sw->next16 ? 2 : 4;
+  sw->head_emb = DevBuf(std::max<uint64_t>(tn * D, 1) * 4);
+  sw->tail_emb = DevBuf(std::max<uint64_t>(tn * D, 1) * 4);
+  sw->row_ptr = DevBuf((tn + 1) * 8);
+  sw->head = DevBuf(std::max<uint64_t>(te, 1) * 4);
+  sw->tail = DevBuf(std::max<uint64_t>(te, 1) * 4);
+  sw->rate = DevBuf(std::max<uint64_t>(te, 1) * 16);
+  sw->sched = DevBuf(std::max<uint64_t>(te, 1) * 16);
+  sw->next = DevBuf(std::max<uint64_t>(te, 1) * nb);
+  sw->active = DevBuf(list_capacity(te) * kShards * 4);
+  sw->counters = DevBuf(2 * kShards * kCounterStride * 4);
+  sw->stats = DevBuf(16);
+  UMAP_HIP(hipMemsetAsync(sw->stats.p, 0, 16, s));
+  sw->fits_host.clear();
+  uint64_t ro = 0, eo = 0;
+  for (const OptimizerState* m : members) {
+    SweepFit f{};
+    f.a = m->a;
+    f.b = m->b;
+    f.m2ab = (-2.0f * m->a) * m->b;
+    f.g2b = (2.0f * m->gamma) * m->b;
+    f.n = (uint32_t)m->n;
+    f.ref_den = (m->semantics & kSemanticsUmapLearnForces) ? 0u : 1u;
+    f.row_off = (uint32_t)ro;
+    f.edge_off = eo;
+    sw->fits_host.push_back(f);
+    if (m->n * D)
+      UMAP_HIP(hipMemcpyAsync(sw->head_emb.as<float>() + ro * D, m->head_emb.p, m->n * D * 4, hipMemcpyDeviceToDevice, s));
+    add_u64_kernel<<<grid_stride_blocks(m->n + 1), 256, 0, s>>>(sw->row_ptr.as<uint64_t>() + ro,
+                                                                 m->row_ptr.as<uint64_t>(), m->n + 1, eo);
+    UMAP_LAUNCH_CHECK();
+    if (m->n_edges) {
+      add_u32_kernel<<<grid_stride_blocks(m->n_edges), 256, 0, s>>>(sw->head.as<uint32_t>() + eo,
+                                                                     m->head.as<uint32_t>(), m->n_edges, (uint32_t)ro);
+      add_u32_kernel<<<grid_stride_blocks(m->n_edges), 256, 0, s>>>(sw->tail.as<uint32_t>() + eo,
+                                                                     m->tail.as<uint32_t>(), m->n_edges, (uint32_t)ro);
+      UMAP_LAUNCH_CHECK();
+      UMAP_HIP(hipMemcpyAsync(sw->rate.as<double2>() + eo, m->rate.p, m->n_edges * 16, hipMemcpyDeviceToDevice, s));
+      UMAP_HIP(hipMemcpyAsync(sw->sched.as<double2>() + eo, m->sched.p, m->n_edges * 16, hipMemcpyDeviceToDevice, s));
+      UMAP_HIP(hipMemcpyAsync(static_cast<char*>(sw->next.p) + eo * nb, m->next.p, m->n_edges * nb,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    ro += m->n;
+    eo += m->n_edges;
+  }
+  sw->fits = DevBuf(sw->fits_host.size() * sizeof(SweepFit));
+  UMAP_HIP(hipMemcpyAsync(sw->fits.p, sw->fits_host.data(), sw->fits_host.size() * sizeof(SweepFit),
+                          hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void sweep_step(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
+  const uint32_t nf = (uint32_t)sw->members.size();
+  for (const OptimizerState* m : sw->members)
+    if (m->current_epoch != sw->current_epoch)
+      throw Error(Err::InvalidArgument, "a sweep member was stepped outside the sweep");
+  for (const OptimizerState* m : sw->members)
+    if (sw->current_epoch + n_epochs > m->total_epochs)  // optimizer.rs:298-304
+      throw Error(Err::Overstep, "Cannot step " + std::to_string(n_epochs) + " epochs: would exceed total_epochs " +
+                                     std::to_string(m->total_epochs) + " (current: " +
+                                     std::to_string(m->current_epoch) + ")");
+  if (n_epochs == 0) return;
+  // per (epoch, fit): alpha = lr * (1 - epoch/total) in f32 (optimizer.rs:317) and the RNG key
+  std::vector<SweepEpoch> tab(n_epochs * nf);
+  for (uint64_t i = 0; i < n_epochs; ++i) {
+    const uint64_t e = sw->current_epoch + i;
+    for (uint32_t f = 0; f < nf; ++f) {
+      const OptimizerState* m = sw->members[f];
+      tab[i * nf + f].alpha = m->initial_alpha * (1.0f - ((float)e / (float)m->total_epochs));
+      tab[i * nf + f].epoch_key = rng_epoch_key(m->seed, e);
+    }
+  }
+  if (sw->fit_epochs.bytes < tab.size() * sizeof(SweepEpoch)) sw->fit_epochs = DevBuf(tab.size() * sizeof(SweepEpoch));
+  UMAP_HIP(hipMemcpyAsync(sw->fit_epochs.p, tab.data(), tab.size() * sizeof(SweepEpoch), hipMemcpyHostToDevice, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  hipEvent_t ev0, ev1;
+  UMAP_HIP(hipEventCreate(&ev0));
+  UMAP_HIP(hipEventCreate(&ev1));
+  UMAP_HIP(hipEventRecord(ev0, s));
+  if (sw->total_e) {
+    if (sw->next16)
+      sweep_run<uint16_t>(sw, n_epochs, s);
+    else
+      sweep_run<uint32_t>(sw, n_epochs, s);
+  }
+  UMAP_HIP(hipEventRecord(ev1, s));
+  UMAP_HIP(hipEventSynchronize(ev1));
+  float ms = 0.0f;
+  UMAP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  sw->last_step_ms = ms;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  // write every member's embedding and schedule back
+  const uint64_t D = sw->dim, nb = sw->next16 ? 2 : 4;
+  for (uint32_t f = 0; f < nf; ++f) {
+    OptimizerState* m = sw->members[f];
+    const SweepFit& fp = sw->fits_host[f];
+    if (m->n * D)
+      UMAP_HIP(hipMemcpyAsync(m->head_emb.p, sw->head_emb.as<float>() + (uint64_t)fp.row_off * D, m->n * D * 4,
+                              hipMemcpyDeviceToDevice, s));
+    if (m->n_edges) {
+      UMAP_HIP(hipMemcpyAsync(m->sched.p, sw->sched.as<double2>() + fp.edge_off, m->n_edges * 16,
+                              hipMemcpyDeviceToDevice, s));
+      UMAP_HIP(hipMemcpyAsync(m->next.p, static_cast<char*>(sw->next.p) + fp.edge_off * nb, m->n_edges * nb,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    m->current_epoch += n_epochs;
+  }
+  sw->current_epoch += n_epochs;
+  UMAP_HIP(hipStreamSynchronize(s));
+}
+
+void sweep_read_stats(const SweepState& sw, uint64_t* pos, uint64_t* neg, hipStream_t s) {
+  uint64_t v[2];
+  UMAP_HIP(hipMemcpyAsync(v, sw.stats.p, 16, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  *pos = v[0];
+  *neg = v[1];
 }
 
 }  // namespace umapgpu

@@ -1,5 +1,8 @@
 // Optimizer state and SGD on the device.
 #pragma once
+#include <utility>
+#include <vector>
+
 #include "graph.hpp"
 
 namespace umapgpu {
@@ -14,6 +17,23 @@ enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2, Hogwild
 // (optimize_layout_euclidean.rs:293, :329).
 constexpr uint32_t kSemanticsSharedBuffer = 1;
 constexpr uint32_t kSemanticsUmapLearnForces = 2;
+
+// One fit of a batched sweep, as the SGD kernels read it.
+struct SweepFit {
+  float a, b, m2ab, g2b;  // curve parameters, (-2a)b, (2 gamma)b
+  uint32_t n;             // the fit's vertices (negative-sample range)
+  uint32_t ref_den;       // 1: reference force denominators
+  uint32_t row_off;       // global row of the fit's vertex 0
+  uint32_t pad;
+  uint64_t edge_off;      // global index of the fit's edge 0
+};
+// A fit's epoch-dependent values for one epoch.
+struct SweepEpoch {
+  float alpha;
+  uint32_t pad;
+  uint64_t epoch_key;
+};
+constexpr uint32_t kMaxSweepFits = 1024;
 
 // Device-resident Optimizer (optimizer.rs:28-56).  Edges are kept in the
 // reference's CSR row order, so an edge's head is its row in `row_ptr`.
@@ -72,5 +92,33 @@ void optimizer_export(const OptimizerState& st, uint32_t* head, uint32_t* tail, 
                       double* epns, double* eonns, float* embedding, hipStream_t s);
 
 void optimizer_read_stats(const OptimizerState& st, uint64_t* pos, uint64_t* neg, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched sweep: many independent fits stepped together.  Their edges and
+// embeddings are concatenated (rows and edges offset per fit) so that one
+// edge-parallel launch per epoch serves every fit; the epoch loop is captured
+// once into a hipGraph per step call.  The members stay owned by the caller;
+// each step writes their embedding and schedule back and advances their epoch.
+struct SweepState {
+  std::vector<OptimizerState*> members;
+  uint64_t dim = 0, total_n = 0, total_e = 0, current_epoch = 0;
+  uint32_t semantics = 0;  // shared-buffer bit, common to all members
+  bool next16 = true;
+  SgdMode mode = SgdMode::Hogwild;
+  bool use_graph = true;
+  uint32_t blocks_per_shard = 16;  // SGD grid = 128 regions x blocks_per_shard (autotuned)
+  bool tuned = false;
+  std::vector<std::pair<uint32_t, float>> tuning;  // (blocks per shard, SGD kernel ms of one epoch)
+  std::vector<SweepFit> fits_host;
+  DevBuf fits, fit_epochs;
+  DevBuf head_emb, tail_emb, row_ptr, head, tail, rate, sched, next, active, counters, stats;
+  double last_step_ms = 0.0;
+  uint64_t graph_nodes = 0;
+};
+
+void sweep_build(SweepState* sw, const std::vector<OptimizerState*>& members, hipStream_t s);
+// Advances every member by n_epochs (all members share the current epoch).
+void sweep_step(SweepState* sw, uint64_t n_epochs, hipStream_t s);
+void sweep_read_stats(const SweepState& sw, uint64_t* pos, uint64_t* neg, hipStream_t s);
 
 }  // namespace umapgpu

@@ -159,6 +159,14 @@ def lib():
             "umapgpu_gather_rows_device": ([P, u64, u64, P, u64, P], i32),
             "umapgpu_memcpy_dtod": ([P, P, u64], i32),
             "umapgpu_pca_device": ([P, u64, u64, u64, P, P], i32),
+            "umapgpu_sweep_create": ([C.POINTER(P), u64, C.POINTER(P)], i32),
+            "umapgpu_sweep_step_epochs": ([P, u64], i32),
+            "umapgpu_sweep_set_mode": ([P, i32], i32),
+            "umapgpu_sweep_set_graph": ([P, i32], i32),
+            "umapgpu_sweep_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], i32),
+            "umapgpu_sweep_stats": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(f64), C.POINTER(u32)], i32),
+            "umapgpu_sweep_tuning_json": ([P, C.c_char_p, u64], u64),
+            "umapgpu_sweep_destroy": ([P], None),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
             "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
@@ -805,6 +813,46 @@ def knn_bruteforce_device(data: DeviceArray, k: int):
     dst = DeviceArray((n, k), np.float32)
     _check(lib().umapgpu_knn_bruteforce_device(P(data.ptr), n, dim, k, P(idx.ptr), P(dst.ptr)))
     return idx, dst
+
+
+class Sweep(_Handle):
+    """Batched sweep of independent optimizers stepped together (one launch
+    per epoch for all of them, epoch loop captured into a hipGraph).  The
+    members must outlive the sweep and are only stepped through it."""
+    _destroy = "umapgpu_sweep_destroy"
+
+    def __init__(self, members):
+        self._members = list(members)
+        arr = (P * len(self._members))(*[m._handle() for m in self._members])
+        out = P()
+        _check(lib().umapgpu_sweep_create(arr, len(self._members), C.byref(out)))
+        super().__init__(out)
+
+    def step_epochs(self, n: int) -> None:
+        _check(lib().umapgpu_sweep_step_epochs(self._handle(), n))
+
+    def set_mode(self, mode: int) -> None:
+        _check(lib().umapgpu_sweep_set_mode(self._handle(), mode))
+
+    def set_graph(self, use_graph: bool) -> None:
+        _check(lib().umapgpu_sweep_set_graph(self._handle(), 1 if use_graph else 0))
+
+    def info(self):
+        f, r, e, c = u64(), u64(), u64(), u64()
+        _check(lib().umapgpu_sweep_info(self._handle(), C.byref(f), C.byref(r), C.byref(e), C.byref(c)))
+        return {"fits": f.value, "rows": r.value, "edges": e.value, "current_epoch": c.value}
+
+    def stats(self):
+        p, q, ms, b = u64(), u64(), f64(), u32()
+        _check(lib().umapgpu_sweep_stats(self._handle(), C.byref(p), C.byref(q), C.byref(ms), C.byref(b)))
+        return {"positive": p.value, "negative": q.value, "last_step_ms": ms.value, "blocks_per_region": b.value}
+
+    def tuning(self):
+        import json
+        n = lib().umapgpu_sweep_tuning_json(self._handle(), None, 0)
+        buf = C.create_string_buffer(int(n))
+        lib().umapgpu_sweep_tuning_json(self._handle(), buf, n)
+        return json.loads(buf.value.decode())
 
 
 @dataclasses.dataclass
