@@ -251,6 +251,77 @@ def wall_section(U, args, c2_state):
     return res
 
 
+def sweep_section(U, args):
+    """64 independent fits (n log-uniform in [5k, 500k], D = 32, 5-50
+    components, k in {10, 15, 30}, min_dist in {0, 0.1, 0.5}, 300 epochs):
+    one batched sweep against the same fits stepped one at a time."""
+    rng = np.random.default_rng(42)
+    fits = []
+    for f in range(args.sweep_fits):
+        n = int(np.exp(rng.uniform(np.log(5_000), np.log(500_000))))
+        comps = int(rng.integers(5, 51))
+        k = int(rng.choice([10, 15, 30]))
+        md = float(rng.choice([0.0, 0.1, 0.5]))
+        fits.append((n, comps, k, md, 100 + f))
+    t = time.perf_counter()
+    prepared = []
+    for n, comps, k, md, seed in fits:
+        x, _ = make_data(n, 32, comps, seed=seed)
+        dx = U.DeviceArray.from_host(x)
+        di, dd = U.knn_bruteforce_device(dx, k)
+        cfg = U.UmapConfig()
+        cfg.graph.n_neighbors = k
+        cfg.manifold.min_dist = md
+        cfg.optimization.n_epochs = 300
+        man = U.learn_manifold_device(cfg, di, dd)
+        di.free()
+        dd.free()
+        dx.free()
+        init = U.DeviceArray.from_host(np.random.default_rng(seed).uniform(-10, 10, (n, 2)).astype(np.float32))
+        prepared.append((man, cfg, init, n, seed))
+    U.synchronize()
+    prep_s = time.perf_counter() - t
+    total_n = sum(f[0] for f in fits)
+    log(f"sweep: {len(fits)} fits, {total_n:,} points, inputs and graphs built in {prep_s:.1f}s")
+
+    def make_opts():
+        return [U.Optimizer.new_device(m.clone(), i.ptr, n, 2, 300, c, U.MetricType.Euclidean, seed=sd)
+                for m, c, i, n, sd in prepared]
+
+    # one at a time
+    opts = make_opts()
+    U.synchronize()
+    t = time.perf_counter()
+    for o in opts:
+        o.step_epochs(300)
+    U.synchronize()
+    seq_s = time.perf_counter() - t
+    seq_updates = sum(o.stats()[0] for o in opts)
+    del opts
+    # batched
+    opts = make_opts()
+    sw = U.Sweep(opts)
+    U.synchronize()
+    t = time.perf_counter()
+    sw.step_epochs(300)
+    U.synchronize()
+    sweep_s = time.perf_counter() - t
+    st = sw.stats()
+    info = sw.info()
+    log(f"sweep: one at a time {seq_s:.3f}s, batched {sweep_s:.3f}s (graph, grid {st['blocks_per_region']} blocks "
+        f"per region), updates {st['positive']}")
+    res = {"workload": f"{len(fits)} fits, n log-uniform in [5k, 500k] (total {total_n:,}), D=32 mixtures of 5-50 "
+                       f"components, k in {{10,15,30}}, min_dist in {{0,0.1,0.5}}, 2-D, 300 epochs each",
+           "rows": info["rows"], "edges": info["edges"], "inputs_and_graphs_s": round(prep_s, 2),
+           "one_at_a_time_s": round(seq_s, 4), "batched_s": round(sweep_s, 4),
+           "batched_edge_updates_per_s": round(st["positive"] / sweep_s, 1),
+           "one_at_a_time_edge_updates_per_s": round(seq_updates / seq_s, 1),
+           "speedup": round(seq_s / sweep_s, 3), "updates_equal": bool(seq_updates == st["positive"]),
+           "autotune": sw.tuning(), "blocks_per_region": st["blocks_per_region"]}
+    del sw, opts, prepared
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +336,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-quality", action="store_true")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-init wall-to-target comparison")
+    ap.add_argument("--skip-sweep", action="store_true", help="skip the batched-sweep comparison")
+    ap.add_argument("--sweep-fits", type=int, default=64)
     ap.add_argument("--wall-n", type=int, default=5_000_000,
                     help="points of the wall-to-target workload (128-D, 100 components on 10-D subspaces)")
     ap.add_argument("--learned-epochs", type=int, default=100)
@@ -432,6 +505,9 @@ def main() -> int:
                                "sample": f"same graph and init, Hogwild epochs 1..{done} of {epochs} "
                                          f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
         log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")
+
+    if not args.skip_sweep:
+        out["sweep"] = sweep_section(U, args)
 
     if not args.skip_learned:
         state = {"dx": dx, "d_idx": d_idx, "d_dst": d_dst, "manifold": manifold, "d_init": d_init,

@@ -281,7 +281,8 @@ UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_devic
  * seed and negative-sample range, so each draws exactly the negatives it
  * would draw alone (the SERIAL mode of a sweep equals each member's own
  * SERIAL run bit for bit).  The SGD grid (edges per workgroup) is picked by
- * timing candidate grids on the first epochs of the first Hogwild call.
+ * timing candidate grids on the first epochs of the first Hogwild call, by
+ * device time per positive update.
  * Members stay owned by the caller and must outlive the sweep; they must
  * share n_components, the current epoch and the buffer semantics, and must
  * not be stepped outside the sweep while it exists.  Each sweep step writes
@@ -303,8 +304,10 @@ UMAPGPU_API umapgpu_status umapgpu_sweep_info(const umapgpu_sweep* s, uint64_t* 
  * device time; the chosen SGD blocks per region (grid = 128 x that). */
 UMAPGPU_API umapgpu_status umapgpu_sweep_stats(const umapgpu_sweep* s, uint64_t* positive, uint64_t* negative,
                                                double* last_step_ms, uint32_t* blocks_per_region);
-/* Autotuner record as JSON ([{"blocks_per_region": b, "ms": t}, ...]); same
- * length convention as umapgpu_manifold_timings_json. */
+/* Autotuner record as JSON ([{"blocks_per_region": b, "ms": t, "updates": u},
+ * ...]: each candidate's epoch time and positive updates; the grid with the
+ * least time per update wins); same length convention as
+ * umapgpu_manifold_timings_json. */
 UMAPGPU_API uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t len);
 UMAPGPU_API void umapgpu_sweep_destroy(umapgpu_sweep* s);
 

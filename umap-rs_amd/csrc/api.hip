@@ -966,8 +966,9 @@ uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t l
   std::string j = "[";
   for (size_t i = 0; i < s->sw.tuning.size(); ++i) {
     if (i) j += ",";
-    j += "{\"blocks_per_region\":" + std::to_string(s->sw.tuning[i].first) +
-         ",\"ms\":" + std::to_string(s->sw.tuning[i].second) + "}";
+    const auto& t = s->sw.tuning[i];
+    j += "{\"blocks_per_region\":" + std::to_string(t.blocks_per_shard) + ",\"ms\":" + std::to_string(t.ms) +
+         ",\"updates\":" + std::to_string(t.updates) + "}";
   }
   j += "]";
   if (buf && len) {

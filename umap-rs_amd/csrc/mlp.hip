@@ -1,22 +1,24 @@
 // Learned-initialisation MLP on bf16 MFMA (gfx950 v_mfma_f32_32x32x16_bf16).
 //
 // One Adam step over a batch of B rows is three launches:
-//   1. mlp_rows_kernel<TRAIN>: a block owns 64 rows.  Layer 1 streams the
+//   1. mlp_rows_kernel<TRAIN, MT>: a block owns 32 MT rows (32 when
+//      training, so that a 4096-row batch spreads over 128 CUs; 64 when
+//      predicting, to halve the weight reads per row).  Layer 1 streams the
 //      normalised input through LDS in 64-column chunks; layers 1 and 2 are
-//      64x256 MFMA tiles (wave w owns columns 64w..64w+63) whose activations
-//      stay in LDS as bf16; layer 3 (256 -> O) is a short f32 dot product.
+//      MFMA tiles (wave w owns columns 64w..64w+63) whose activations stay
+//      in LDS as bf16; layer 3 (256 -> O) is a short f32 dot product.
 //      The same block then runs the data backward: dY from the squared error,
 //      dH2 = (dY W3) * [H2 > 0], dH1 = (dH2 W2) * [H1 > 0] on MFMA.  Inputs,
 //      activations and their gradients leave the block feature-major
 //      ([feature][row], bf16) so that the weight gradients read them with
 //      unit stride.
-//   2. mlp_wgrad_kernel: dW = dH^T H as 64x64 MFMA tiles, split over 512-row
+//   2. mlp_wgrad_kernel: dW = dH^T H as 64x64 MFMA tiles, split over 256-row
 //      slices of the batch into per-slice partials (no atomics, so a step is
 //      deterministic); one extra block per slice does the biases and W3.
 //   3. mlp_adam_kernel: sums the slice partials in order, applies Adam to the
 //      f32 master weights and refreshes the bf16 copies (W1, W2 and W2^T, the
 //      last feeding the dH1 product).
-// Prediction is mlp_rows_kernel<false> over all rows.
+// Prediction is mlp_rows_kernel<false, 2> over all rows.
 //
 // MFMA operand layout (32x32x16): lane l holds A[l%32][8(l/32)..+8] and
 // B[8(l/32)..+8][l%32]; accumulator element v of lane l is
@@ -40,11 +42,12 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kH = kMlpHidden;
-constexpr int kR = 64;           // rows per block
+constexpr int kRTrain = 32;      // rows per block when training (more blocks for one batch)
+constexpr int kRPredict = 64;    // rows per block when predicting (fewer weight reads per row)
 constexpr int kKC = 64;          // layer-1 input columns per LDS chunk
 constexpr int kLdX = kKC + 8;    // LDS row strides (bf16), padded against bank conflicts
 constexpr int kLdH = kH + 8;
-constexpr int kKS = 512;         // batch rows per weight-gradient slice
+constexpr int kKS = 256;         // batch rows per weight-gradient slice
 constexpr int kThreads = 256;
 
 // f32 -> bf16, round to nearest even (NaN stays NaN).
@@ -89,19 +92,20 @@ struct RowsArgs {
   float* loss_part;        // per block
 };
 
-// acc[2][2] += A(64 x K, LDS, row stride lda) * B^T where B rows are
+// acc[MT][2] += A(32MT x K, LDS, row stride lda) * B^T where B rows are
 // columns n0..n0+63 of the layer (global, row stride ldw), k in [k0, k0+kn).
-__device__ inline void tile_mfma(f32x16 (&acc)[2][2], const uint16_t* a_lds, int lda, int a_k0,
+template <int MT>
+__device__ inline void tile_mfma(f32x16 (&acc)[MT][2], const uint16_t* a_lds, int lda, int a_k0,
                                  const uint16_t* w, uint64_t ldw, int n0, int w_k0, int kn, int lane) {
   const int r = lane & 31, h = (lane >> 5) * 8;
   for (int kk = 0; kk < kn; kk += 16) {
-    bf16x8 af[2], bw[2];
+    bf16x8 af[MT], bw[2];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) bw[nt] = ld8(w + (uint64_t)(n0 + nt * 32 + r) * ldw + w_k0 + kk + h);
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) af[mt] = ld8(a_lds + (mt * 32 + r) * lda + a_k0 + kk + h);
+    for (int mt = 0; mt < MT; ++mt) af[mt] = ld8(a_lds + (mt * 32 + r) * lda + a_k0 + kk + h);
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mfma(af[mt], bw[nt], acc[mt][nt]);
   }
@@ -109,11 +113,11 @@ __device__ inline void tile_mfma(f32x16 (&acc)[2][2], const uint16_t* a_lds, int
 
 // Bias + ReLU epilogue into an LDS bf16 tile, and (training) the same values
 // feature-major to global: the 4 rows v%4 = 0..3 of a lane are contiguous.
-template <bool TRAIN>
-__device__ inline void relu_out(const f32x16 (&acc)[2][2], const float* __restrict__ bias, uint16_t* lds,
+template <bool TRAIN, int MT>
+__device__ inline void relu_out(const f32x16 (&acc)[MT][2], const float* __restrict__ bias, uint16_t* lds,
                                 uint16_t* __restrict__ gt, uint32_t ldb, uint32_t m0, int n0, int lane) {
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
       const int col = n0 + nt * 32 + (lane & 31);
@@ -135,22 +139,25 @@ __device__ inline void relu_out(const f32x16 (&acc)[2][2], const float* __restri
     }
 }
 
-template <bool TRAIN>
+// MT: 32-row MFMA tiles per block (the block owns R = 32 MT rows).
+template <bool TRAIN, int MT>
 __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
-  __shared__ __align__(16) uint16_t xs[kR * kLdX];
-  __shared__ __align__(16) uint16_t h1[kR * kLdH];
-  __shared__ __align__(16) uint16_t h2[kR * kLdH];
-  __shared__ __align__(16) uint16_t dh2[TRAIN ? kR * kLdH : 8];
+  constexpr int R = 32 * MT;
+  constexpr int TPR = kThreads / R;  // lanes per row in the output layer
+  __shared__ __align__(16) uint16_t xs[R * kLdX];
+  __shared__ __align__(16) uint16_t h1[R * kLdH];
+  __shared__ __align__(16) uint16_t h2[R * kLdH];
+  __shared__ __align__(16) uint16_t dh2[TRAIN ? R * kLdH : 8];
   __shared__ float w3[kMlpMaxOut * kH];
-  __shared__ float dys[kR * kMlpMaxOut];
-  __shared__ uint32_t rid[kR];
+  __shared__ float dys[R * kMlpMaxOut];
+  __shared__ uint32_t rid[R];
   __shared__ float red[kThreads / 64];
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t m0 = blockIdx.x * kR;
+  const uint32_t m0 = blockIdx.x * R;
   const int n0 = wave * 64;
   const uint32_t O = a.O;
-  if (t < kR) {
+  if (t < R) {
     const uint32_t m = m0 + t;
     rid[t] = m < a.nrows ? (a.ids ? a.ids[m] : m) : 0xffffffffu;
   }
@@ -158,20 +165,21 @@ __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
   __syncthreads();
 
   // ---- layer 1
-  f32x16 acc[2][2];
+  f32x16 acc[MT][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
   for (uint32_t k0 = 0; k0 < a.Dp; k0 += kKC) {
-    {  // stage: thread owns column c of 16 rows
+    {  // stage: thread owns column c of R/4 rows
+      constexpr int RG = R / 4;
       const uint32_t c = k0 + (t & 63);
-      const int r0 = (t >> 6) * 16;
+      const int r0 = (t >> 6) * RG;
       const bool cin = c < a.D;
       const float mu = cin ? a.xmean[c] : 0.0f, iv = cin ? a.xinv[c] : 0.0f;
-      uint16_t q[16];
+      uint16_t q[RG];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
+      for (int e = 0; e < RG; ++e) {
         const uint32_t row = rid[r0 + e];
         float x = 0.0f;
         if (cin && row != 0xffffffffu) x = (a.X[(uint64_t)row * a.D + c] - mu) * iv;
@@ -179,49 +187,46 @@ __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
         xs[(r0 + e) * kLdX + (t & 63)] = q[e];
       }
       if (TRAIN) {
-        u32x4 lo, hi;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          lo[e] = (uint32_t)q[2 * e] | ((uint32_t)q[2 * e + 1] << 16);
-          hi[e] = (uint32_t)q[8 + 2 * e] | ((uint32_t)q[8 + 2 * e + 1] << 16);
+        for (int g = 0; g < RG; g += 8) {
+          u32x4 pk;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) pk[e] = (uint32_t)q[g + 2 * e] | ((uint32_t)q[g + 2 * e + 1] << 16);
+          *(u32x4*)(a.xt + (uint64_t)c * a.ldb + m0 + r0 + g) = pk;
         }
-        uint16_t* dst = a.xt + (uint64_t)c * a.ldb + m0 + r0;
-        *(u32x4*)dst = lo;
-        *(u32x4*)(dst + 8) = hi;
       }
     }
     __syncthreads();
-    tile_mfma(acc, xs, kLdX, 0, a.w1b, a.Dp, n0, (int)k0, kKC, lane);
+    tile_mfma<MT>(acc, xs, kLdX, 0, a.w1b, a.Dp, n0, (int)k0, kKC, lane);
     __syncthreads();
   }
-  relu_out<TRAIN>(acc, a.params + a.off_b1, h1, a.h1t, a.ldb, m0, n0, lane);
+  relu_out<TRAIN, MT>(acc, a.params + a.off_b1, h1, a.h1t, a.ldb, m0, n0, lane);
   __syncthreads();
 
   // ---- layer 2
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
-  tile_mfma(acc, h1, kLdH, 0, a.w2b, kH, n0, 0, kH, lane);
-  relu_out<TRAIN>(acc, a.params + a.off_b2, h2, a.h2t, a.ldb, m0, n0, lane);
+  tile_mfma<MT>(acc, h1, kLdH, 0, a.w2b, kH, n0, 0, kH, lane);
+  relu_out<TRAIN, MT>(acc, a.params + a.off_b2, h2, a.h2t, a.ldb, m0, n0, lane);
   __syncthreads();
 
-  // ---- layer 3: four lanes per row, each a quarter of the 256 inputs
-  const int r = t >> 2, qtr = t & 3;
+  // ---- layer 3: TPR lanes per row, each a contiguous share of the 256 inputs
+  const int r = t / TPR, qtr = t % TPR;
   float y[kMlpMaxOut];
 #pragma unroll
   for (int o = 0; o < kMlpMaxOut; ++o) y[o] = 0.0f;
-  for (int j = qtr * 64; j < qtr * 64 + 64; ++j) {
+  for (int j = qtr * (kH / TPR); j < (qtr + 1) * (kH / TPR); ++j) {
     const float hv = bf2f(h2[r * kLdH + j]);
 #pragma unroll
     for (int o = 0; o < kMlpMaxOut; ++o)
       if (o < (int)O) y[o] = y[o] + hv * w3[o * kH + j];
   }
 #pragma unroll
-  for (int o = 0; o < kMlpMaxOut; ++o) {
-    y[o] = y[o] + __shfl_xor(y[o], 1, 64);
-    y[o] = y[o] + __shfl_xor(y[o], 2, 64);
-  }
+  for (int o = 0; o < kMlpMaxOut; ++o)
+#pragma unroll
+    for (int off = 1; off < TPR; off <<= 1) y[o] = y[o] + __shfl_xor(y[o], off, 64);
   const uint32_t row = rid[r];
   if (!TRAIN) {
     if (qtr == 0 && row != 0xffffffffu)
@@ -253,7 +258,7 @@ __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
   // ---- dH2 = (dY W3) * [H2 > 0]: thread owns column j, 8 rows per store
   {
     const int j = t;
-    for (int r0 = 0; r0 < kR; r0 += 8) {
+    for (int r0 = 0; r0 < R; r0 += 8) {
       uint16_t q[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -273,12 +278,12 @@ __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
 
   // ---- dH1 = (dH2 W2) * [H1 > 0]: B operand rows are columns of W2 (W2^T rows)
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < MT; ++i)
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj) acc[i][jj] = f32x16{};
-  tile_mfma(acc, dh2, kLdH, 0, a.w2tb, kH, n0, 0, kH, lane);
+  tile_mfma<MT>(acc, dh2, kLdH, 0, a.w2tb, kH, n0, 0, kH, lane);
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
       const int col = n0 + nt * 32 + (lane & 31);
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(kThreads) void mlp_rows_kernel(RowsArgs a) {
 struct WgradArgs {
   const uint16_t *xt, *h1t, *h2t, *dh1t, *dh2t;
   const float* dyt;
-  uint32_t ldb, kc;        // row stride; batch columns in use (multiple of 64)
+  uint32_t ldb, kc;        // row stride; batch columns in use (multiple of 32)
   uint32_t D, Dp, O;
   uint32_t w1_tiles;       // 4 * Dp / 64
   float* part;             // [slices x P]
@@ -322,9 +327,17 @@ __global__ __launch_bounds__(kThreads) void mlp_wgrad_kernel(WgradArgs a) {
     const uint16_t* B = (is_w2 ? a.h1t : a.xt) + (uint64_t)(i0 + (lane & 31)) * a.ldb + (lane >> 5) * 8;
     f32x16 acc0 = {}, acc1 = {};
     uint32_t k = kb;
-    for (; k + 32 <= ke; k += 32) {
-      acc0 = mfma(ld8(A + k), ld8(B + k), acc0);
-      acc1 = mfma(ld8(A + k + 16), ld8(B + k + 16), acc1);
+    for (; k + 64 <= ke; k += 64) {  // eight loads in flight, then four MFMAs
+      bf16x8 av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        av[u] = ld8(A + k + 16 * u);
+        bv[u] = ld8(B + k + 16 * u);
+      }
+      acc0 = mfma(av[0], bv[0], acc0);
+      acc1 = mfma(av[1], bv[1], acc1);
+      acc0 = mfma(av[2], bv[2], acc0);
+      acc1 = mfma(av[3], bv[3], acc1);
     }
     for (; k < ke; k += 16) acc0 = mfma(ld8(A + k), ld8(B + k), acc0);
     const uint32_t i = i0 + (lane & 31);
@@ -531,8 +544,8 @@ uint32_t batch_gradients(MlpState* st, const float* X, const float* Y, const uin
   a.T = Y;
   a.dy_scale = (float)(2.0 / ((double)nb * (double)L.O));
   a.loss_part = loss_part;
-  const unsigned tiles = (unsigned)ceil_div(nb, kR);
-  mlp_rows_kernel<true><<<tiles, kThreads, 0, s>>>(a);
+  const unsigned tiles = (unsigned)ceil_div(nb, kRTrain);
+  mlp_rows_kernel<true, kRTrain / 32><<<tiles, kThreads, 0, s>>>(a);
   UMAP_LAUNCH_CHECK();
   WgradArgs w{};
   w.xt = st->xt.as<uint16_t>();
@@ -542,7 +555,7 @@ uint32_t batch_gradients(MlpState* st, const float* X, const float* Y, const uin
   w.dh2t = st->dh2t.as<uint16_t>();
   w.dyt = st->dyt.as<float>();
   w.ldb = (uint32_t)st->bcap;
-  w.kc = tiles * kR;
+  w.kc = tiles * kRTrain;
   w.D = (uint32_t)L.D;
   w.Dp = (uint32_t)L.Dp;
   w.O = (uint32_t)L.O;
@@ -615,7 +628,7 @@ void mlp_create(const MlpConfig& c, MlpState* st, hipStream_t s) {
   st->dh2t.alloc((uint64_t)kH * st->bcap * 2);
   st->dyt.alloc(L.O * st->bcap * 4);
   st->part.alloc(ceil_div(st->bcap, kKS) * L.total * 4);
-  st->loss_part.alloc(ceil_div(st->bcap, kR) * 4);
+  st->loss_part.alloc(ceil_div(st->bcap, kRTrain) * 4);
   st->rows.alloc(st->bcap * 4);
   // the padded input columns of W1's bf16 copy stay zero
   UMAP_HIP(hipMemsetAsync(st->w1b.p, 0, st->w1b.bytes, s));
@@ -693,7 +706,7 @@ double mlp_gradients(MlpState* st, const float* X, const float* Y, uint64_t n, f
   if (n < 1 || n > st->bcap) throw Error(Err::InvalidArgument, "gradient batch must hold 1..batch rows");
   const uint32_t slices = batch_gradients(st, X, Y, nullptr, n, st->loss_part.as<float>(), s);
   launch_adam(st, slices, false, st->grad.as<float>(), s);
-  const uint64_t tiles = ceil_div(n, kR);
+  const uint64_t tiles = ceil_div(n, kRTrain);
   std::vector<float> lp(tiles);
   UMAP_HIP(hipMemcpyAsync(lp.data(), st->loss_part.p, tiles * 4, hipMemcpyDeviceToHost, s));
   UMAP_HIP(hipMemcpyAsync(grad_host, st->grad.p, st->lay.total * 4, hipMemcpyDeviceToHost, s));
@@ -708,7 +721,7 @@ double mlp_train(MlpState* st, const float* X, const float* Y, uint64_t n, uint6
   check_rows(*st, n);
   const uint64_t B = st->bcap;
   const uint64_t batches = ceil_div(n, B);
-  const uint64_t tiles_per = ceil_div(B, kR);
+  const uint64_t tiles_per = ceil_div(B, kRTrain);
   DevBuf ids(n * 4), losses(batches * tiles_per * 4);
   std::vector<uint32_t> perm(n);
   std::vector<float> lp(batches * tiles_per);
@@ -739,7 +752,7 @@ double mlp_train(MlpState* st, const float* X, const float* Y, uint64_t n, uint6
       double tot = 0.0;
       for (uint64_t b = 0; b < batches; ++b) {
         const uint64_t nb = std::min<uint64_t>(B, n - b * B);
-        for (uint64_t tt = 0; tt < ceil_div(nb, kR); ++tt) tot += lp[b * tiles_per + tt];
+        for (uint64_t tt = 0; tt < ceil_div(nb, kRTrain); ++tt) tot += lp[b * tiles_per + tt];
       }
       last = tot / ((double)n * (double)st->lay.O);
     } else {
@@ -762,7 +775,7 @@ void mlp_predict(const MlpState& st, const float* X, uint64_t n, float* Y, hipSt
   if (n == 0) return;
   RowsArgs a = rows_args(st, X, n);
   a.Y = Y;
-  mlp_rows_kernel<false><<<(unsigned)ceil_div(n, kR), kThreads, 0, s>>>(a);
+  mlp_rows_kernel<false, kRPredict / 32><<<(unsigned)ceil_div(n, kRPredict), kThreads, 0, s>>>(a);
   UMAP_LAUNCH_CHECK();
   UMAP_HIP(hipStreamSynchronize(s));
 }

@@ -1130,6 +1130,7 @@ void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t 
 }
 
 // Candidate SGD grids of the autotuner: 128 regions x blocks per region.
+// (sweep_read_stats is declared in optimize.hpp.)
 constexpr uint32_t kTuneBps[] = {4, 8, 16, 32};
 
 template <typename NextT>
@@ -1144,29 +1145,40 @@ void sweep_run(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
   uint64_t done = 0;
   const SweepEpoch* fe = sw->fit_epochs.as<SweepEpoch>();
   // autotune: the first epochs of the first Hogwild call run one candidate
-  // grid each (every candidate computes a valid epoch), timed on the stream
-  if (!sw->tuned && sw->mode != SgdMode::Serial && n_epochs >= sizeof(kTuneBps) / sizeof(kTuneBps[0]) + 1) {
+  // grid each (every candidate computes a valid epoch), scored by device time
+  // per positive update since epochs differ in how many edges are active; an
+  // epoch without updates (epoch 0) does not score its candidate, which then
+  // gets the next epoch
+  if (!sw->tuned && sw->mode != SgdMode::Serial) {
     hipEvent_t a, b;
     UMAP_HIP(hipEventCreate(&a));
     UMAP_HIP(hipEventCreate(&b));
     float best = 0.0f;
-    for (uint32_t bps : kTuneBps) {
+    const size_t ncand = sizeof(kTuneBps) / sizeof(kTuneBps[0]);
+    size_t c = 0;
+    while (c < ncand && done + 1 < n_epochs) {
+      uint64_t p0 = 0, p1 = 0, q = 0;
+      sweep_read_stats(*sw, &p0, &q, s);
       UMAP_HIP(hipEventRecord(a, s));
-      sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, bps, s);
+      sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, kTuneBps[c], s);
       UMAP_HIP(hipEventRecord(b, s));
       UMAP_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
       UMAP_HIP(hipEventElapsedTime(&ms, a, b));
-      sw->tuning.push_back({bps, ms});
-      if (best == 0.0f || ms < best) {
-        best = ms;
-        sw->blocks_per_shard = bps;
-      }
+      sweep_read_stats(*sw, &p1, &q, s);
       ++done;
+      if (p1 == p0) continue;
+      const float per = ms / (float)(p1 - p0);
+      sw->tuning.push_back({kTuneBps[c], ms, p1 - p0});
+      if (best == 0.0f || per < best) {
+        best = per;
+        sw->blocks_per_shard = kTuneBps[c];
+      }
+      ++c;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    sw->tuned = true;
+    sw->tuned = c == ncand;
   }
   if (done == n_epochs) return;
   if (!sw->use_graph) {

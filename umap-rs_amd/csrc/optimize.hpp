@@ -108,7 +108,12 @@ struct SweepState {
   bool use_graph = true;
   uint32_t blocks_per_shard = 16;  // SGD grid = 128 regions x blocks_per_shard (autotuned)
   bool tuned = false;
-  std::vector<std::pair<uint32_t, float>> tuning;  // (blocks per shard, SGD kernel ms of one epoch)
+  struct Trial {
+    uint32_t blocks_per_shard;
+    float ms;           // device time of the epoch (list + SGD kernels)
+    uint64_t updates;   // positive updates in that epoch
+  };
+  std::vector<Trial> tuning;
   std::vector<SweepFit> fits_host;
   DevBuf fits, fit_epochs;
   DevBuf head_emb, tail_emb, row_ptr, head, tail, rate, sched, next, active, counters, stats;
