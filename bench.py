@@ -111,6 +111,60 @@ def knn_by_component(U, dx, x, lab, k):
     return idx, dst, worst
 
 
+def make_data_background(n: int, dim: int, n_comp: int, frac_bg: float, seed: int = 0):
+    """The mixture of make_data for (1 - frac_bg) n points plus frac_bg n points
+    uniform over the mixture's bounding box (label -1)."""
+    n_bg = int(round(n * frac_bg))
+    x, lab = make_data(n - n_bg, dim, n_comp, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    lo, hi = x.min(0), x.max(0)
+    bg = (lo + rng.random((n_bg, dim), dtype=np.float32) * (hi - lo)).astype(np.float32)
+    return np.concatenate([x, bg]), np.concatenate([lab, np.full(n_bg, -1)])
+
+
+def knn_mixture_background(U, dx, x, lab, k):
+    """kNN for make_data_background: mixture rows by component (exact when the
+    certificate is < 1: every component's k-th neighbour distance lies below
+    the least distance another component or any background point can reach);
+    background rows among the background only (not certified)."""
+    n = x.shape[0]
+    idx = np.empty((n, k), np.uint32)
+    dst = np.empty((n, k), np.float32)
+    mix = lab >= 0
+    comps = int(lab[mix].max()) + 1
+    order = np.argsort(np.where(mix, lab, comps), kind="stable")
+    bounds = np.searchsorted(np.where(mix, lab, comps)[order], np.arange(comps + 2))
+    means = np.zeros((comps, x.shape[1]))
+    radius = np.zeros(comps)
+    kth = np.zeros(comps)
+    for c in range(comps + 1):
+        rows = order[bounds[c]:bounds[c + 1]].astype(np.uint32)
+        d_rows = U.DeviceArray.from_host(rows)
+        sub = U.gather_rows_device(dx, d_rows)
+        li, ld = U.knn_bruteforce_device(sub, k)
+        li, ld = li.to_host(), ld.to_host()
+        idx[rows] = rows[li]
+        dst[rows] = ld
+        sub.free()
+        d_rows.free()
+        if c < comps:
+            kth[c] = float(ld[:, k - 1].max())
+            xc = x[rows].astype(np.float64)
+            means[c] = xc.mean(0)
+            radius[c] = float(np.sqrt(((xc - means[c]) ** 2).sum(1)).max())
+    gap = np.sqrt(((means[:, None, :] - means[None, :, :]) ** 2).sum(-1)) - radius[:, None] - radius[None, :]
+    np.fill_diagonal(gap, np.inf)
+    bgx = x[~mix].astype(np.float64)
+    bg_min = np.full(comps, np.inf)
+    m2 = (means ** 2).sum(1)
+    for s0 in range(0, bgx.shape[0], 1 << 16):
+        b = bgx[s0:s0 + (1 << 16)]
+        d2 = (b ** 2).sum(1)[:, None] + m2[None, :] - 2.0 * b @ means.T
+        bg_min = np.minimum(bg_min, np.sqrt(np.maximum(d2, 0.0)).min(0))
+    worst = float(np.max(kth / np.minimum(gap.min(1), bg_min - radius)))
+    return idx, dst, worst
+
+
 def bytes_per_update(dim: int, nsr: int) -> int:
     return 8 + 16 * dim + 4 * dim * nsr
 
@@ -322,6 +376,93 @@ def sweep_section(U, args):
     return res
 
 
+def mlp_profile(U) -> int:
+    """Learned-init network kernels at the 5M x 128 workload's shapes: 3
+    training epochs over 200k rows (batch 4096) and 3 predictions of 5M rows
+    (for a kernel trace)."""
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(200_000, 128)).astype(np.float32)
+    y = (np.stack([np.sin(x[:, 0]), np.cos(x[:, 1])], 1) * 5).astype(np.float32)
+    dx, dy = U.DeviceArray.from_host(x), U.DeviceArray.from_host(y)
+    m = U.Mlp(U.MlpConfig(in_dim=128, out_dim=2, batch=4096))
+    m.fit_normalization(dx, dy)
+    t = time.perf_counter()
+    m.train(dx, dy, 3)
+    U.synchronize()
+    steps, ms = m.stats()
+    flop = 3 * 2 * (128 * 256 + 256 * 256 + 256 * 2) * 4096 * steps
+    log(f"mlp: {steps} Adam steps in {time.perf_counter() - t:.4f}s ({ms:.2f} ms device), "
+        f"{flop / (ms * 1e-3) / 1e12:.1f} TFLOP/s")
+    big = U.DeviceArray.from_host(rng.normal(size=(5_000_000, 128)).astype(np.float32))
+    for _ in range(3):
+        t = time.perf_counter()
+        m.predict(big)
+        U.synchronize()
+        dt = time.perf_counter() - t
+        log(f"mlp: predict 5M rows {dt * 1e3:.2f} ms, "
+            f"{5e6 * 2 * (128 * 256 + 256 * 256 + 256 * 2) / dt / 1e12:.1f} TFLOP/s")
+    return 0
+
+
+def large_graph_section(U, args):
+    """n = 20M, D = 64: 200-component mixture plus 5% uniform background, kNN
+    k = 30, symmetrised (~1.2B CSR entries), 2-D, min_dist 0.05, 500 epochs:
+    graph build seconds and SGD edge-updates/s at scale."""
+    n, dim, k, epochs = args.large_n, 64, 30, 500
+    t = time.perf_counter()
+    x, lab = make_data_background(n, dim, 200, 0.05, seed=5)
+    log(f"large graph: n={n:,} data generated in {time.perf_counter() - t:.1f}s")
+    dx = U.DeviceArray.from_host(x)
+    t = time.perf_counter()
+    idx, dst, worst = knn_mixture_background(U, dx, x, lab, k)
+    knn_s = time.perf_counter() - t
+    log(f"large graph: kNN {knn_s:.1f}s, mixture-row certificate {worst:.4f}")
+    del x
+    dx.free()
+    d_idx, d_dst = U.DeviceArray.from_host(idx), U.DeviceArray.from_host(dst)
+    del idx, dst
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = k
+    cfg.manifold.min_dist = 0.05
+    cfg.optimization.n_epochs = epochs
+    U.synchronize()
+    t = time.perf_counter()
+    manifold = U.learn_manifold_device(cfg, d_idx, d_dst)
+    U.synchronize()
+    graph_s = time.perf_counter() - t
+    gt = manifold.timings()
+    log(f"large graph: learn_manifold {graph_s:.3f}s, nnz={gt['nnz']:,} (directed {gt['nnz_directed']:,})")
+    d_idx.free()
+    d_dst.free()
+    graph_bytes = 8 * n * k + 8 * (n + 1) + 8 * gt["nnz"]
+    d_init = U.DeviceArray.from_host(np.random.default_rng(1).uniform(-10, 10, (n, 2)).astype(np.float32))
+    t = time.perf_counter()
+    opt = U.Optimizer.new_device(manifold, d_init.ptr, n, 2, epochs, cfg, U.MetricType.Euclidean, seed=2)
+    U.synchronize()
+    new_s = time.perf_counter() - t
+    t = time.perf_counter()
+    opt.step_epochs(epochs)
+    U.synchronize()
+    sgd_s = time.perf_counter() - t
+    p, q, ms = opt.stats()
+    rate = p / (ms / 1e3)
+    bpu = bytes_per_update(2, cfg.optimization.negative_sample_rate)
+    log(f"large graph: Optimizer::new {new_s:.2f}s, SGD {sgd_s:.2f}s ({ms:.1f} ms device), {p:,} updates, "
+        f"{rate:.4g} updates/s")
+    return {"workload": f"n={n:,} D=64: 200-component mixture + 5% uniform background, kNN k={k}, symmetrised, "
+                        f"2-D, min_dist 0.05, {epochs} epochs, Hogwild",
+            "knn_s": round(knn_s, 2), "knn_mixture_rows_certificate": round(worst, 5),
+            "knn_note": "mixture rows exact by component when the certificate is < 1; background rows' neighbours "
+                        "are taken among the background only",
+            "graph_build_s": round(graph_s, 4), "graph_nnz": int(gt["nnz"]), "graph_nnz_directed": int(gt["nnz_directed"]),
+            "graph_stages_ms": {st["name"]: round(st["ms"], 2) for st in gt["stages"]},
+            "graph_algorithmic_GBps": round(graph_bytes / graph_s / 1e9, 1),
+            "optimizer_new_s": round(new_s, 3), "edges_after_filter": int(opt.info()[0]),
+            "sgd_s": round(sgd_s, 3), "sgd_device_ms": round(ms, 1), "positive_updates": int(p),
+            "edge_updates_per_s": round(rate, 1),
+            "sgd_roofline_frac": round(bpu * rate / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,12 +478,17 @@ def main() -> int:
     ap.add_argument("--skip-quality", action="store_true")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-init wall-to-target comparison")
     ap.add_argument("--skip-sweep", action="store_true", help="skip the batched-sweep comparison")
+    ap.add_argument("--large-graph", action="store_true",
+                    help="also run the 20M-point k=30 large-graph workload (graph build + SGD at ~1.2B edges)")
+    ap.add_argument("--large-n", type=int, default=20_000_000)
     ap.add_argument("--sweep-fits", type=int, default=64)
     ap.add_argument("--wall-n", type=int, default=5_000_000,
                     help="points of the wall-to-target workload (128-D, 100 components on 10-D subspaces)")
     ap.add_argument("--learned-epochs", type=int, default=100)
     ap.add_argument("--subsample", type=int, default=200_000)
     ap.add_argument("--mlp-epochs", type=int, default=30)
+    ap.add_argument("--mlp-profile", action="store_true",
+                    help="only run the learned-init network's kernels at the 5M x 128 shapes (for a kernel trace)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (scales profiler traffic counters)")
     args = ap.parse_args()
@@ -356,6 +502,8 @@ def main() -> int:
     if U.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.mlp_profile:
+        return mlp_profile(U)
 
     n, dim, k, epochs = args.n, args.dim, args.k, args.epochs
     cfg = U.UmapConfig()
@@ -508,6 +656,9 @@ def main() -> int:
 
     if not args.skip_sweep:
         out["sweep"] = sweep_section(U, args)
+
+    if args.large_graph:
+        out["large_graph"] = large_graph_section(U, args)
 
     if not args.skip_learned:
         state = {"dx": dx, "d_idx": d_idx, "d_dst": d_dst, "manifold": manifold, "d_init": d_init,
