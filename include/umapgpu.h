@@ -46,11 +46,24 @@ typedef enum umapgpu_metric_type { UMAPGPU_METRIC_EUCLIDEAN = 0, UMAPGPU_METRIC_
 
 /* SGD execution modes of Optimizer::step_epochs (an addition: the reference
  * always runs its Rayon Hogwild loop, optimizer.rs:337).
- *   HOGWILD: edge-parallel; each epoch compacts its active edges, one lane
- *            runs one edge's positive and negative samples in order and adds
- *            its change to the head row with f32 atomics; tail-snapshot
- *            updates are f32 atomics too (no lost updates).  Default.
- *   HOGWILD_RACY: as HOGWILD with plain racy tail stores, as the reference.
+ *   HOGWILD (default): the parallel epoch.  When every edge (j,k) has a twin
+ *            (k,j) with a bit-identical schedule (the symmetric graphs
+ *            learn_manifold builds), it runs row-owner synchronous: every lane
+ *            reads rows from a copy of both arrays taken at the start of the
+ *            epoch, runs its edge's attractive and negative samples, and also
+ *            evaluates the twin edge's tail side from the same copy, so both
+ *            ends of an edge move by the same gradient as in the reference;
+ *            each row is then written once by the block that owns it.  No
+ *            atomics, and the result does not depend on scheduling: runs are
+ *            bit-reproducible.  Otherwise it runs as HOGWILD_WINDOW_ATOMIC.
+ *   HOGWILD_WINDOW_ATOMIC: the same row windows, but each edge's tail side
+ *            is applied to the live snapshot at once (compare-and-swap) and
+ *            head changes are added with f32 atomics.
+ *   HOGWILD_EDGE_ATOMIC: a list kernel compacts the active edges, then one
+ *            lane per edge with compare-and-swap tail updates and f32 atomic
+ *            head updates.
+ *   HOGWILD_RACY: as HOGWILD_EDGE_ATOMIC with plain racy tail stores, as
+ *            the reference.
  *   SERIAL: the reference's edge order on one lane (deterministic,
  *           bit-identical to a sequential CPU run).
  *   HOGWILD_ROWS: one lane per head row walks its edges in order with the
@@ -59,7 +72,9 @@ typedef enum umapgpu_sgd_mode {
   UMAPGPU_SGD_HOGWILD = 0,
   UMAPGPU_SGD_SERIAL = 1,
   UMAPGPU_SGD_HOGWILD_RACY = 2,
-  UMAPGPU_SGD_HOGWILD_ROWS = 3
+  UMAPGPU_SGD_HOGWILD_ROWS = 3,
+  UMAPGPU_SGD_HOGWILD_EDGE_ATOMIC = 4,
+  UMAPGPU_SGD_HOGWILD_WINDOW_ATOMIC = 5
 } umapgpu_sgd_mode;
 
 /* UmapConfig (config.rs:206-235) with ManifoldParams (:9-52), GraphParams

@@ -220,14 +220,15 @@ class OptimizerState:
         return self.head.size
 
     def step_epochs(self, n_epochs: int, mode: str = "serial"):
-        """Optimizer::step_epochs (optimizer.rs:297-366); mode 'serial' or 'hogwild'."""
+        """Optimizer::step_epochs (optimizer.rs:297-366); mode 'serial', 'hogwild', or 'sync' (the
+        build's row-owner synchronous epoch, restated for bit-level checks of the device default)."""
         if self.current_epoch + n_epochs > self.total_epochs:
             raise ValueError(f"Cannot step {n_epochs} epochs: would exceed total_epochs {self.total_epochs} "
                              f"(current: {self.current_epoch})")
         lib().orc_step_epochs(_ptr(self.embedding), self.n, self.dim, _ptr(self.head), _ptr(self.tail),
                               self.n_edges, _ptr(self.eps), _ptr(self.eons), _ptr(self.epns), _ptr(self.eonns),
                               self.a, self.b, self.gamma, self.initial_alpha, self.current_epoch, n_epochs,
-                              self.total_epochs, self.seed, 0 if mode == "serial" else 1, _ptr(self.counts),
+                              self.total_epochs, self.seed, {"serial": 0, "hogwild": 1, "sync": 2}[mode], _ptr(self.counts),
                               self.semantics)
         self.current_epoch += n_epochs
 

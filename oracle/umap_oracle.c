@@ -32,6 +32,7 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+#include <stdio.h>
 #endif
 
 #define ORC_API __attribute__((visibility("default")))
@@ -614,8 +615,121 @@ ORC_API void orc_sgd_epoch_hogwild(float *head_emb, float *tail_emb, uint64_t di
   }
 }
 
+/* One epoch of the build's row-owner synchronous mode (the device default on
+ * symmetric schedules).  Not a reference function: it restates the device
+ * algorithm so that its output can be checked bit for bit.  Every read comes
+ * from h0/t0, copies of the head and tail arrays taken at the start of the
+ * epoch.  For each active edge i = (j,k) of row j, in edge order:
+ *   own sample  h = h0[j] + clip(gc(h0[j], t0[k]) * (h0[j] - t0[k])) * alpha,
+ *               then its negative samples on h against t0 (rs:278-341);
+ *   twin tail   dt = -clip(gc(h0[k], t0[j]) * (h0[k] - t0[j])) * alpha, the
+ *               tail side of the twin edge (k,j) (rs:284-306), which the
+ *               owner of row k evaluates from the same values for its head;
+ * and the row is written once: head[j] = h0[j] + ((0 + dh_1) + dh_2 ...),
+ * tail[j] = ((t0[j] + dt_1) + dt_2) ...; with one shared array,
+ * head[j] = (h0[j] + sum dh) + (t - t0[j]).                                 */
+ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, uint64_t dim, const uint32_t *head,
+                                const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices, const double *eps,
+                                double *eons, const double *epns, double *eonns, float a, float b, float gamma,
+                                float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts) {
+  if (dim > 16) {
+    fprintf(stderr, "orc_sgd_epoch_sync: dim %llu > 16\n", (unsigned long long)dim);
+    abort();
+  }
+  const uint64_t nel = (uint64_t)n_vertices * dim;
+  float *h0 = (float *)malloc((nel ? nel : 1) * sizeof(float));
+  float *t0 = shared ? h0 : (float *)malloc((nel ? nel : 1) * sizeof(float));
+  memcpy(h0, head_emb, nel * sizeof(float));
+  if (!shared) memcpy(t0, tail_emb, nel * sizeof(float));
+  uint64_t *rp = (uint64_t *)calloc((uint64_t)n_vertices + 1, sizeof(uint64_t));
+  for (uint64_t i = 0; i < n_edges; i++) rp[head[i] + 1]++;
+  for (uint64_t r = 0; r < n_vertices; r++) rp[r + 1] += rp[r];
+  const uint64_t ekey = orc_rng_epoch_key(seed, epoch);
+  const float m2ab = (-2.0f * a) * b, g2b = (2.0f * gamma) * b;
+  uint64_t np = 0, nn = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : np, nn)
+  for (uint64_t j = 0; j < n_vertices; j++) {
+    float acc[16], t[16], h[16], dif[16];
+    int any = 0;
+    for (uint64_t d = 0; d < dim; d++) {
+      acc[d] = 0.0f;
+      t[d] = t0[j * dim + d];
+    }
+    for (uint64_t i = rp[j]; i < rp[j + 1]; i++) {
+      if (!(eons[i] <= (double)epoch)) continue;
+      any = 1;
+      const uint32_t k = tail[i];
+      eons[i] = eons[i] + eps[i];
+      const uint64_t n_neg = sat_u64(((double)epoch - eonns[i]) / epns[i]);
+      eonns[i] = eonns[i] + (double)n_neg * epns[i];
+      float d2 = 0.0f;
+      for (uint64_t d = 0; d < dim; d++) {
+        dif[d] = h0[j * dim + d] - t0[(uint64_t)k * dim + d];
+        d2 = d2 + dif[d] * dif[d];
+      }
+      float gc = 0.0f;
+      if (d2 > 0.0f) {
+        const float pb = powf(d2, b);
+        gc = (m2ab * pb) / d2;
+        gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
+      }
+      for (uint64_t d = 0; d < dim; d++) h[d] = h0[j * dim + d] + clip4(gc * dif[d]) * alpha;
+      const uint64_t ik = orc_rng_edge_key(ekey, i);
+      for (uint64_t p = 0; p < n_neg; p++) {
+        const uint32_t kk = orc_rng_index(ik, p, n_vertices);
+        if (kk == j) continue;
+        float dd = 0.0f;
+        for (uint64_t d = 0; d < dim; d++) {
+          dif[d] = h[d] - t0[(uint64_t)kk * dim + d];
+          dd = dd + dif[d] * dif[d];
+        }
+        float g2 = 0.0f;
+        if (dd > 0.0f) {
+          const float pb = powf(dd, b);
+          g2 = g2b / ((0.001f + dd) * ((a * pb) * (g_ref_den ? dd : 1.0f) + 1.0f));
+        }
+        if (g2 > 0.0f)
+          for (uint64_t d = 0; d < dim; d++) h[d] = h[d] + clip4(g2 * dif[d]) * alpha;
+      }
+      for (uint64_t d = 0; d < dim; d++) acc[d] = acc[d] + (h[d] - h0[j * dim + d]);
+      d2 = 0.0f;
+      for (uint64_t d = 0; d < dim; d++) {
+        dif[d] = h0[(uint64_t)k * dim + d] - t0[j * dim + d];
+        d2 = d2 + dif[d] * dif[d];
+      }
+      gc = 0.0f;
+      if (d2 > 0.0f) {
+        const float pb = powf(d2, b);
+        gc = (m2ab * pb) / d2;
+        gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
+      }
+      for (uint64_t d = 0; d < dim; d++) t[d] = t[d] + (-(clip4(gc * dif[d]) * alpha));
+      np += 1;
+      nn += n_neg;
+    }
+    if (!any) continue;
+    for (uint64_t d = 0; d < dim; d++) {
+      const float hv = h0[j * dim + d] + acc[d];
+      if (shared) {
+        head_emb[j * dim + d] = hv + (t[d] - t0[j * dim + d]);
+      } else {
+        head_emb[j * dim + d] = hv;
+        tail_emb[j * dim + d] = t[d];
+      }
+    }
+  }
+  free(rp);
+  if (!shared) free(t0);
+  free(h0);
+  if (counts) {
+    counts[0] += np;
+    counts[1] += nn;
+  }
+}
+
 /* Optimizer::step_epochs (optimizer.rs:297-366): tail snapshot per call,
- * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild.
+ * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild, 2 = the
+ * build's synchronous row-owner epoch (orc_sgd_epoch_sync).
  * semantics: bit 1 = one shared array instead of the tail snapshot, bit 2 =
  * umap-learn's force denominators (both opt-in departures; 0 = reference). */
 ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const uint32_t *head, const uint32_t *tail,
@@ -633,6 +747,9 @@ ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const u
     if (mode == 0)
       orc_sgd_epoch_serial(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
                            gamma, alpha, e, seed, counts);
+    else if (mode == 2)
+      orc_sgd_epoch_sync(embedding, copy, shared, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a,
+                         b, gamma, alpha, e, seed, counts);
     else
       orc_sgd_epoch_hogwild(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
                             gamma, alpha, e, seed, counts);

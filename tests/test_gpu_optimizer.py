@@ -9,6 +9,9 @@
   positive updates and negative samples and the schedule arrays equal the
   sequential run exactly; the embedding is compared statistically (fuzzy
   cross-entropy within 5% of the CPU Hogwild run).
+* The default parallel epoch on symmetric schedules (row-owner synchronous)
+  is deterministic: its embedding is bit-identical to the CPU restatement of
+  that epoch (oracle mode "sync") under every semantics setting.
 """
 import numpy as np
 import pytest
@@ -110,13 +113,16 @@ def test_serial_trajectory_bit_identical(gpu, oracle):
     assert (p, q) == tuple(int(c) for c in ost.counts)
 
 
-@pytest.mark.parametrize("mode", ["hogwild", "hogwild_racy", "hogwild_rows"])
+_MODES = {"hogwild": "SGD_HOGWILD", "hogwild_racy": "SGD_HOGWILD_RACY", "hogwild_rows": "SGD_HOGWILD_ROWS",
+          "hogwild_edge_atomic": "SGD_HOGWILD_EDGE_ATOMIC", "hogwild_window_atomic": "SGD_HOGWILD_WINDOW_ATOMIC"}
+
+
+@pytest.mark.parametrize("mode", list(_MODES))
 def test_hogwild_schedule_exact_and_quality(gpu, oracle, mode):
     import umap_eval as E
     total = 200
     x, idx, cfg, opt, ost = _setup(gpu, oracle, n=3000, total=total, seed=3)
-    opt.set_mode({"hogwild": gpu.SGD_HOGWILD, "hogwild_racy": gpu.SGD_HOGWILD_RACY,
-                  "hogwild_rows": gpu.SGD_HOGWILD_ROWS}[mode])
+    opt.set_mode(getattr(gpu, _MODES[mode]))
     y0 = opt.embedding()
     opt.step_epochs(total)
     ost.step_epochs(total, "hogwild")
@@ -249,3 +255,65 @@ def test_opt_in_semantics_restore_cluster_structure(gpu, oracle):
     print(f"[semantics] purity reference={res[0]:.3f} umap-learn={res[3]:.3f}", flush=True)
     assert res[3] > 0.95
     assert res[3] > res[0] + 0.2
+
+
+@pytest.mark.parametrize("semantics", [0, 1, 2, 3])
+@pytest.mark.parametrize("n_components", [2, 3])
+def test_default_epoch_bit_identical_to_sync_restatement(gpu, oracle, semantics, n_components):
+    """The default epoch on a symmetric graph (row-owner synchronous) against
+    the CPU restatement of the same epoch: embedding, schedule and counts
+    bit-identical after two step_epochs calls (the tail snapshot is retaken
+    per call), for each force denominator and tail-array setting."""
+    _, _, _, opt, ost = _setup(gpu, oracle, n=3000, total=60, seed=6, n_components=n_components)
+    opt.set_semantics(semantics)
+    ost.semantics = semantics
+    for chunk in (25, 35):
+        opt.step_epochs(chunk)
+        ost.step_epochs(chunk, "sync")
+        y = opt.embedding()
+        bad = int((y.view(np.uint32) != ost.embedding.view(np.uint32)).sum())
+        print(f"[sync] semantics={semantics} d={n_components} epoch={ost.current_epoch} differing={bad} "
+              f"max|diff|={float(np.abs(y - ost.embedding).max()):.3g}", flush=True)
+        assert bad == 0
+    p, q, _ = opt.stats()
+    assert (p, q) == tuple(int(c) for c in ost.counts)
+    st = opt.export_state()
+    assert np.array_equal(st["epoch_of_next_sample"], ost.eons)
+    assert np.array_equal(st["epoch_of_next_negative_sample"], ost.eonns)
+
+
+def test_default_epoch_is_reproducible(gpu, oracle):
+    """No atomics and fixed summation orders: two runs agree bit for bit."""
+    _, _, _, a, _ = _setup(gpu, oracle, n=20000, total=40, seed=2)
+    _, _, _, b, _ = _setup(gpu, oracle, n=20000, total=40, seed=2)
+    a.step_epochs(40)
+    b.step_epochs(40)
+    assert np.array_equal(a.embedding().view(np.uint32), b.embedding().view(np.uint32))
+
+
+def test_directed_graph_falls_back_and_keeps_quality(gpu, oracle):
+    """symmetrize = false leaves edges without a twin, so the default epoch
+    applies tail updates edge by edge (window-atomic path): exact counts and
+    cross-entropy within 5% of the CPU Hogwild run."""
+    import umap_eval as E
+    n, total = 3000, 150
+    x, _ = gmm(n, 16, 6, seed=12)
+    idx, d = knn_exact(x, 15)
+    cfg = gpu.UmapConfig()
+    cfg.graph.n_neighbors = 15
+    cfg.graph.symmetrize = False
+    init = uniform_init(n, 2, seed=112)
+    m = gpu.Umap(cfg).learn_manifold(x, idx, d)
+    om = oracle.learn_manifold(idx, d, symmetrize=False)
+    opt = gpu.Optimizer.new(m, init, total, cfg, gpu.MetricType.Euclidean, seed=12)
+    ost = oracle.OptimizerState(om, init, total, seed=12)
+    opt.step_epochs(total)
+    ost.step_epochs(total, "hogwild")
+    p, q, _ = opt.stats()
+    assert (p, q) == tuple(int(c) for c in ost.counts)
+    g = opt.manifold().graph()
+    a, b = opt.manifold().curve_params()
+    ce_gpu = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), opt.embedding(), a, b)
+    ce_cpu = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), ost.embedding, a, b)
+    print(f"[directed] CE gpu={ce_gpu:.1f} cpu={ce_cpu:.1f}", flush=True)
+    assert abs(ce_gpu - ce_cpu) / ce_cpu < 0.05

@@ -663,7 +663,7 @@ umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* o, uint6
 
 umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* o, int32_t mode) {
   return guarded([&] {
-    if (mode < 0 || mode > 3) panic("unknown SGD mode");
+    if (mode < 0 || mode > 5) panic("unknown SGD mode");
     o->st.mode = (SgdMode)mode;
   });
 }

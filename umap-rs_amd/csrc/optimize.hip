@@ -400,7 +400,10 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64
 // XCD's L2 then only holds its eighth of the head embedding.
 constexpr uint32_t kShards = 128;
 constexpr uint32_t kCounterStride = 32;  // u32 words between counters (128 B)
-constexpr uint32_t kBlocksPerShard = 16;  // SGD blocks walking one region
+#ifndef UMAP_SGD_BPS
+#define UMAP_SGD_BPS 16
+#endif
+constexpr uint32_t kBlocksPerShard = UMAP_SGD_BPS;  // SGD blocks walking one region
 #ifndef UMAP_SGD_PREFETCH
 #define UMAP_SGD_PREFETCH 6
 #endif
@@ -672,6 +675,421 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
   wave_add_stats(npos, nneg, stats);
 }
 
+// ---------------------------------------------------------------------------
+// Row-owner Hogwild (the default when every edge's twin has a bit-identical
+// schedule, which holds for the symmetric graphs learn_manifold builds).
+//
+// Device-scope atomics on this memory are carried out at the memory side, so
+// the per-edge tail update of sgd_edges_kernel (a 64-bit compare-and-swap on
+// a random row of the snapshot) costs a full round trip per edge.  With
+// symmetric schedules every active edge (j,k) has an active twin (k,j) in the
+// same epoch, and the twin's tail-side update lands on T[j]:
+//     T[j] -= clip(gc(|H[k] - T[j]|^2) * (H[k] - T[j])) * alpha
+// (optimize_layout_euclidean.rs:284-306 with the roles of j and k swapped).
+// So the lane that runs edge (j,k) also evaluates that twin update, reading
+// H[k] next to T[k].  Every write of an epoch then lands on the lane's own
+// head row j, for both arrays: a row's active edges are listed contiguously
+// and run inside one wave, whose run-tail lane stores H[j] and T[j] with
+// plain stores.  Only a row whose run crosses a wave boundary falls back to
+// f32 atomics.  Each active edge still makes exactly one head update and one
+// tail update per active epoch, with the counts and schedule of the
+// reference; what changes is which Hogwild interleaving occurs.
+
+// Step 0 (Optimizer::new): list blocks hold whole rows, those whose first
+// edge falls in [b*W, (b+1)*W).
+__global__ void list_blocks_kernel(const uint64_t* __restrict__ row_ptr, uint64_t n, uint64_t window,
+                                   uint64_t n_blocks, uint32_t* __restrict__ first_row) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > n_blocks) return;
+  if (b == n_blocks) {
+    first_row[b] = (uint32_t)n;
+    return;
+  }
+  const uint64_t target = b * window;
+  uint64_t lo = 0, hi = n;  // first r in [0, n) with row_ptr[r] >= target, n if none
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (row_ptr[mid] < target) lo = mid + 1; else hi = mid;
+  }
+  first_row[b] = (uint32_t)lo;
+}
+
+// Every edge (j,k) has a twin (k,j) (found by binary search in row k, whose
+// tails are sorted) with bit-identical {eps, epns} and {eons, eonns}.
+__global__ void twin_check_kernel(const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ head,
+                                  const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
+                                  const double2* __restrict__ sched, uint64_t n_edges, unsigned int* __restrict__ bad) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_edges) return;
+  const uint32_t j = head[e], k = tail[e];
+  uint64_t lo = row_ptr[k], hi = row_ptr[k + 1];
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (tail[mid] < j) lo = mid + 1; else hi = mid;
+  }
+  bool ok = lo < row_ptr[k + 1] && tail[lo] == j;
+  if (ok) {
+    const double2 r0 = rate[e], r1 = rate[lo], s0 = sched[e], s1 = sched[lo];
+    ok = __double_as_longlong(r0.x) == __double_as_longlong(r1.x) &&
+         __double_as_longlong(r0.y) == __double_as_longlong(r1.y) &&
+         __double_as_longlong(s0.x) == __double_as_longlong(s1.x) &&
+         __double_as_longlong(s0.y) == __double_as_longlong(s1.y);
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// Step 1 (per epoch, one kernel): block b owns the whole rows of window
+// w(b).  Per pass of 1024 edges it compacts the window's active edges into
+// LDS in edge order (a row's edges stay contiguous), then runs them 256 at a
+// time, one lane per edge:
+//   own update   H[j] += attraction to T[k] + repulsions from T[k'] (:278-341),
+//                 from the row's value at the start of the epoch;
+//   twin update  the lane also reads H[k] for the tail side of the twin edge
+//                 (k,j), T[j] -= clip(gc * (H[k] - T[j])) * alpha (:284-306).
+// The last lane of each row's run then adds the row's head changes and applies
+// its twin updates to T[j] one after another in edge order, as the
+// reference's threads apply the tail updates of a row one after another; a
+// run cut by the end of a chunk is carried into the next.  The row is
+// written once, by its owner: no list, no counters, no atomics on the
+// embedding.  Windows are dealt to XCDs in eighths (block b runs on XCD
+// b % 8), so each XCD's L2 keeps its own rows from one epoch to the next.
+constexpr uint32_t kFusedPass = 1024;     // edges compacted per pass (4 a thread)
+constexpr uint32_t kFusedRowsLds = 2048;  // row pointers staged in LDS (else read from memory)
+
+// Reads of another row and the owner's writes go through the device's
+// coherence point (agent-scope relaxed atomics: sc1 loads and stores), so a
+// row written by its owner on one XCD is seen by lanes on every other XCD in
+// the same epoch, as the reference's threads see each other's stores.
+__device__ inline float remote_load(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void owner_store(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A row whose run goes on in the next chunk.
+template <int D>
+struct RowCarry {
+  uint32_t j;
+  float h0[D], t0[D];  // the row at the start of the epoch
+  float dh[D];         // head change so far
+  float t[D];          // tail value so far
+};
+
+template <int D, bool SHARED>
+__device__ inline void store_row(float* H, float* T, uint32_t j, const float* h0, const float* dh, const float* t0,
+                                 const float* t) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (SHARED) {
+      owner_store(H + (uint64_t)j * D + d, (h0[d] + dh[d]) + (t[d] - t0[d]));
+    } else {
+      owner_store(H + (uint64_t)j * D + d, h0[d] + dh[d]);
+      owner_store(T + (uint64_t)j * D + d, t[d]);
+    }
+  }
+}
+
+// The tail side of one edge, applied at once (optimize_layout_euclidean.rs:303):
+// T[k] -= g * alpha as one 64-bit compare-and-swap when D = 2 (both
+// coordinates together), else an f32 atomic per coordinate.
+template <int D>
+__device__ inline void tail_sub(float* tp, const float* tv, const float* ga) {
+  if constexpr (D == 2) {
+    unsigned long long* p64 = reinterpret_cast<unsigned long long*>(tp);
+    unsigned long long expect =
+        ((unsigned long long)__float_as_uint(tv[1]) << 32) | (unsigned long long)__float_as_uint(tv[0]);
+    for (;;) {
+      const float x0 = __uint_as_float((uint32_t)expect) - ga[0];
+      const float x1 = __uint_as_float((uint32_t)(expect >> 32)) - ga[1];
+      const unsigned long long want =
+          ((unsigned long long)__float_as_uint(x1) << 32) | (unsigned long long)__float_as_uint(x0);
+      const unsigned long long got = atomicCAS(p64, expect, want);
+      if (got == expect) break;
+      expect = got;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) __hip_atomic_fetch_add(tp + d, -ga[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// A row's run is complete.  TWIN: its owner writes H[j] and T[j].  Otherwise
+// the tail sides were applied edge by edge (T[j] may also receive tail
+// updates of other rows' edges), so the head change is added atomically.
+template <int D, bool SHARED, bool TWIN>
+__device__ inline void finish_row(float* H, float* T, uint32_t j, const float* h0, const float* dh, const float* t0,
+                                  const float* t) {
+  if (TWIN) {
+    store_row<D, SHARED>(H, T, j, h0, dh, t0, t);
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      __hip_atomic_fetch_add(H + (uint64_t)j * D + d, dh[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int D, typename NextT, bool SHARED, bool TWIN>
+__global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
+    float* H, float* T, const float* __restrict__ H0, const float* __restrict__ T0,
+    const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ first_row,
+    uint64_t n_windows, const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
+    double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
+  __shared__ uint64_t s_exp2[32];
+  __shared__ double s_log2[32];
+  __shared__ uint64_t s_ptr[kFusedRowsLds + 1];
+  __shared__ uint32_t s_e[kFusedPass], s_j[kFusedPass];
+  __shared__ float s_dh[256][D], s_hk[256][D];  // per lane: head change, twin tail change
+  __shared__ uint32_t s_cnt[4];
+  __shared__ RowCarry<D> s_carry[2];
+  if (threadIdx.x < 32) {
+    const LibmTables c = libm_default_tables();
+    s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
+    s_log2[threadIdx.x] = c.log2[threadIdx.x];
+  }
+  // block -> window: XCD x = b % 8 takes the x-th eighth of the windows
+  const uint32_t x = blockIdx.x % kXcds, q8 = blockIdx.x / kXcds;
+  const uint64_t w_lo = (n_windows * x) / kXcds, w_hi = (n_windows * (x + 1)) / kXcds;
+  const uint64_t win = w_lo + q8;
+  if (win >= w_hi) return;  // block-uniform, before any barrier
+  const uint32_t r0 = first_row[win], r1 = first_row[win + 1];
+  const uint64_t e0 = row_ptr[r0], e1 = row_ptr[r1];
+  if (e0 == e1) return;  // block-uniform
+  const uint32_t nr = r1 - r0;
+  const bool ptr_lds = nr <= kFusedRowsLds;
+  if (ptr_lds)
+    for (uint32_t q = threadIdx.x; q <= nr; q += blockDim.x) s_ptr[q] = row_ptr[r0 + q];
+  if (threadIdx.x == 0) s_carry[0].j = s_carry[1].j = 0xffffffffu;
+  __syncthreads();
+  const LibmTables tb{s_exp2, s_log2};
+  const LaneFit F{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
+  const NextT ep = (NextT)A.epoch;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t npos = 0, nneg = 0;
+  uint32_t chunk_no = 0;
+  for (uint64_t pb = e0 & ~3ull; pb < e1; pb += kFusedPass) {  // block-uniform
+    // ---- compact this pass's active edges into s_e / s_j (edge order)
+    const uint64_t ef = pb + 4ull * threadIdx.x;
+    uint32_t mask = 0;
+    if (ef < e1 && ef + 4 > e0) {
+      NextT v[4];
+      if constexpr (sizeof(NextT) == 2) {
+        const unsigned long long qv = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(next + ef));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = (NextT)((qv >> (16 * t)) & 0xffffu);
+      } else {
+        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 qv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(next + ef));
+        v[0] = qv.x; v[1] = qv.y; v[2] = qv.z; v[3] = qv.w;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint64_t e = ef + t;
+        if (e >= e0 && e < e1 && v[t] <= ep) mask |= 1u << t;
+      }
+    }
+    const uint32_t c = (uint32_t)__popc(mask);
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, (unsigned)off, 64);
+      if (lane >= off) incl += u;
+    }
+    if (lane == 63) s_cnt[w] = incl;
+    __syncthreads();
+    const uint32_t m = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (mask) {
+      uint32_t pos = incl - c;
+      for (int q = 0; q < w; ++q) pos += s_cnt[q];
+      const uint64_t efirst = ef + (uint64_t)__ffs(mask) - 1;
+      uint32_t lo = 0, hi = nr;  // the row q with ptr[q] <= efirst < ptr[q+1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t pm = ptr_lds ? s_ptr[mid] : row_ptr[r0 + mid];
+        if (pm <= efirst) lo = mid; else hi = mid;
+      }
+      uint32_t q = lo;
+      uint64_t nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (!((mask >> t) & 1u)) continue;
+        const uint64_t e = ef + t;
+        while (nstart <= e) {
+          ++q;
+          nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
+        }
+        s_e[pos] = (uint32_t)e;
+        s_j[pos] = r0 + q;
+        ++pos;
+      }
+    }
+    __syncthreads();
+    // ---- run the active edges, 256 at a time
+    for (uint32_t cb = 0; cb < m; cb += 256, ++chunk_no) {  // block-uniform
+      const uint32_t pi = cb + threadIdx.x;
+      const bool valid = pi < m;
+      uint32_t j = 0xffffffffu;
+      float h0[D], t0[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) h0[d] = t0[d] = 0.0f;
+      if (valid) {
+        const uint32_t e = s_e[pi];
+        j = s_j[pi];
+        const uint32_t k = __builtin_nontemporal_load(tail + e);
+        double2 sc, rt;
+        sc.x = __builtin_nontemporal_load(&sched[e].x);
+        sc.y = __builtin_nontemporal_load(&sched[e].y);
+        rt.x = __builtin_nontemporal_load(&rate[e].x);
+        rt.y = __builtin_nontemporal_load(&rate[e].y);
+        sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
+        const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
+        sc.y = sc.y + (double)nn * rt.y;                             // :343
+        __builtin_nontemporal_store(sc.x, &sched[e].x);
+        __builtin_nontemporal_store(sc.y, &sched[e].y);
+        __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
+        const uint64_t ekey = rng_edge_key(F.epoch_key, e);
+        uint32_t self_mask = 0;
+        float tn[kPrefetch][D];
+#pragma unroll
+        for (int q = 0; q < kPrefetch; ++q) {
+          const uint32_t kq = rng_index(ekey, (uint64_t)q, F.n);
+          self_mask |= (kq == j) ? (1u << q) : 0u;
+          if ((uint32_t)q < nn) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) tn[q][d] = TWIN ? T0[(uint64_t)kq * D + d] : T[(uint64_t)kq * D + d];
+          }
+        }
+        float tk[D], hk[D], h[D], diff[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          if (TWIN) {  // every read from the epoch-start snapshot
+            h0[d] = H0[(uint64_t)j * D + d];
+            tk[d] = T0[(uint64_t)k * D + d];
+            t0[d] = T0[(uint64_t)j * D + d];
+            hk[d] = H0[(uint64_t)k * D + d];
+          } else {
+            h0[d] = H[(uint64_t)j * D + d];
+            tk[d] = remote_load(T + (uint64_t)k * D + d);
+            hk[d] = 0.0f;
+          }
+        }
+        {  // own attractive sample (:278-306)
+          float d2 = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            diff[d] = h0[d] - tk[d];
+            d2 = d2 + diff[d] * diff[d];
+          }
+          const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+          float ga[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            ga[d] = clip4(gc * diff[d]) * F.alpha;
+            h[d] = h0[d] + ga[d];
+          }
+          if (!TWIN) tail_sub<D>(T + (uint64_t)k * D, tk, ga);  // this edge's tail side, now
+        }
+        for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
+          bool skip;
+          float o[D];
+          if (p < (uint32_t)kPrefetch) {
+            skip = (self_mask >> p) & 1u;
+#pragma unroll
+            for (int q = 0; q < kPrefetch; ++q)
+              if ((uint32_t)q == p) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) o[d] = tn[q][d];
+              }
+          } else {
+            const uint32_t kp = rng_index(ekey, p, F.n);
+            skip = (kp == j);
+#pragma unroll
+            for (int d = 0; d < D; ++d) o[d] = TWIN ? T0[(uint64_t)kp * D + d] : T[(uint64_t)kp * D + d];
+          }
+          if (skip) continue;
+          float d2 = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            diff[d] = h[d] - o[d];
+            d2 = d2 + diff[d] * diff[d];
+          }
+          const float g2 = (d2 > 0.0f) ? repel_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+          if (g2 > 0.0f) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * F.alpha;
+          }
+        }
+        float dt[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dt[d] = 0.0f;
+        if (TWIN) {  // the twin edge (k,j)'s tail side, from the same values its head side uses
+          float d2 = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            diff[d] = hk[d] - t0[d];
+            d2 = d2 + diff[d] * diff[d];
+          }
+          const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * F.alpha);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          s_dh[threadIdx.x][d] = h[d] - h0[d];
+          s_hk[threadIdx.x][d] = dt[d];
+        }
+        ++npos;
+        nneg += nn;
+      }
+      __syncthreads();
+      // the chunk's last position holds the run that may go on in the next
+      // chunk: it becomes the carry instead of being written
+      const uint32_t last = (m - cb >= 256) ? 255u : (m - cb - 1);
+      const uint32_t par = chunk_no & 1u;
+      const RowCarry<D>& cin = s_carry[par];
+      if (threadIdx.x == 0 && cin.j != 0xffffffffu && cin.j != j)  // the carry ends before this chunk
+        finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
+      if (valid && (threadIdx.x == last || s_j[pi + 1] != j)) {
+        uint32_t p = threadIdx.x;  // the run's first position in this chunk
+        while (p > 0 && s_j[cb + p - 1] == j) --p;
+        const bool cont = (p == 0) && (cin.j == j);
+        float dh[D], t[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dh[d] = cont ? cin.dh[d] : 0.0f;
+          t[d] = cont ? cin.t[d] : t0[d];
+        }
+        for (uint32_t q = p; q <= threadIdx.x; ++q) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dh[d] = dh[d] + s_dh[q][d];
+            t[d] = t[d] + s_hk[q][d];
+          }
+        }
+        if (threadIdx.x == last) {
+          RowCarry<D>& co = s_carry[par ^ 1u];
+          co.j = j;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            co.h0[d] = h0[d];
+            co.t0[d] = t0[d];
+            co.dh[d] = dh[d];
+            co.t[d] = t[d];
+          }
+        } else {
+          finish_row<D, SHARED, TWIN>(H, T, j, h0, dh, t0, t);
+        }
+      }
+      __syncthreads();  // s_dh / s_hk / the carry are rewritten by the next chunk
+    }
+  }
+  if (threadIdx.x == 0) {  // the last pending run
+    const RowCarry<D>& cin = s_carry[chunk_no & 1u];
+    if (cin.j != 0xffffffffu) finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
+  }
+  wave_add_stats(npos, nneg, stats);
+}
+
 __global__ void add_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n,
                                uint32_t add) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -781,6 +1199,9 @@ void launch_rows(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   UMAP_LAUNCH_CHECK();
 }
 
+template <typename NextT, bool TWIN>
+void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s);
+
 template <typename NextT>
 void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   switch (st->mode) {
@@ -801,9 +1222,23 @@ void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
       else
         launch_rows<NextT, false>(st, A, s);
       break;
-    default:
+    case SgdMode::HogwildEdgeAtomic:
       if (st->dim <= 8)
         launch_edges<NextT, true>(st, A, s);
+      else
+        launch_rows<NextT, true>(st, A, s);
+      break;
+    case SgdMode::HogwildWindowAtomic:
+      if (st->dim <= 8)
+        launch_owner<NextT, false>(st, A, s);
+      else
+        launch_rows<NextT, true>(st, A, s);
+      break;
+    default:
+      if (st->dim <= 8 && st->twin_ok)
+        launch_owner<NextT, true>(st, A, s);
+      else if (st->dim <= 8)
+        launch_owner<NextT, false>(st, A, s);
       else
         launch_rows<NextT, true>(st, A, s);
   }
@@ -829,7 +1264,8 @@ void build_perm(OptimizerState* st, hipStream_t s) {
 
 void alloc_next(OptimizerState* st) {
   st->next16 = st->total_epochs <= 65535;
-  st->next = DevBuf(std::max<uint64_t>(st->n_edges, 1) * (st->next16 ? 2 : 4));
+  // padded to whole groups of 8 (the list kernel reads 8 counters at a time)
+  st->next = DevBuf((ceil_div(st->n_edges, 8) * 8 + 8) * (st->next16 ? 2 : 4));
 }
 
 // Per-edge head rows and the active-list buffers of the edge-parallel epoch.
@@ -844,6 +1280,98 @@ void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
     fill_head_kernel<<<grid_for(st->n, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), st->n, st->head.as<uint32_t>());
     UMAP_LAUNCH_CHECK();
   }
+}
+
+// Edges per row-owner window (whole rows whose first edge falls in it).
+// Larger windows amortise a block's start (row pointers, activity scan)
+// over more chunks; measured at 1M points: 2048 edges 1339 us/epoch, 4096
+// 938, 8192 865.  Small graphs use smaller windows so that there are still
+// about 4096 blocks.
+uint64_t owner_window(uint64_t n_edges) {
+  uint64_t w = 256;
+  while (w < 8192 && w * 4096 < n_edges) w <<= 1;
+  return w;
+}
+
+// Row-owner Hogwild set-up: the twin check, then the windows (structure
+// only; fixed for the optimizer).
+void setup_owner(OptimizerState* st, hipStream_t s) {
+  st->twin_ok = false;
+  st->n_lblocks = 0;
+  const uint64_t E = st->n_edges, n = st->n;
+  if (E == 0 || n == 0 || n > 0xffffffffull) return;
+  st->list_window = owner_window(E);
+  st->n_lblocks = ceil_div(E, st->list_window);
+  if (st->n_lblocks > 0xffffffffull / kXcds) throw Error(Err::Unsupported, "too many edges for the SGD windows");
+  st->lblock_row = DevBuf((st->n_lblocks + 1) * 4);
+  list_blocks_kernel<<<grid_for(st->n_lblocks + 1, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), n, st->list_window,
+                                                                       st->n_lblocks, st->lblock_row.as<uint32_t>());
+  UMAP_LAUNCH_CHECK();
+  {
+    DevBuf bad(4);
+    UMAP_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+    twin_check_kernel<<<grid_for(E, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), st->head.as<uint32_t>(),
+                                                         st->tail.as<uint32_t>(), st->rate.as<double2>(),
+                                                         st->sched.as<double2>(), E, bad.as<unsigned int>());
+    UMAP_LAUNCH_CHECK();
+    uint32_t hb = 1;
+    UMAP_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+    UMAP_HIP(hipStreamSynchronize(s));
+    st->twin_ok = (hb == 0);
+  }
+}
+
+template <typename NextT, bool TWIN>
+void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  float* H = st->head_emb.as<float>();
+  float* T = tail_buffer(st);
+  const bool shared = (st->semantics & kSemanticsSharedBuffer) != 0;
+  const unsigned grid = (unsigned)(kXcds * ceil_div(st->n_lblocks, kXcds));
+  const uint64_t* rp = st->row_ptr.as<uint64_t>();
+  const uint32_t* fr = st->lblock_row.as<uint32_t>();
+  const uint32_t* tl = st->tail.as<uint32_t>();
+  const double2* rt = st->rate.as<double2>();
+  double2* sc = st->sched.as<double2>();
+  NextT* nx = st->next.as<NextT>();
+  unsigned long long* stt = st->stats.as<unsigned long long>();
+  const uint64_t nw = st->n_lblocks;
+  const float* H0 = nullptr;
+  const float* T0 = nullptr;
+  if (TWIN) {  // epoch-start snapshots of both arrays
+    const uint64_t nel = st->n * st->dim;
+    if (st->snap_h.bytes < nel * 4) st->snap_h = DevBuf(nel * 4);
+    copy_f32_kernel<<<(unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096), 256, 0, s>>>(st->snap_h.as<float>(), H, nel);
+    UMAP_LAUNCH_CHECK();
+    H0 = st->snap_h.as<float>();
+    T0 = H0;
+    if (!shared) {
+      if (st->snap_t.bytes < nel * 4) st->snap_t = DevBuf(nel * 4);
+      copy_f32_kernel<<<(unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096), 256, 0, s>>>(st->snap_t.as<float>(), T, nel);
+      UMAP_LAUNCH_CHECK();
+      T0 = st->snap_t.as<float>();
+    }
+  }
+#define UMAP_FUSED_CASE(DD)                                                                            \
+  case DD:                                                                                             \
+    if (shared)                                                                                        \
+      sgd_fused_kernel<DD, NextT, true, TWIN><<<grid, 256, 0, s>>>(H, T, H0, T0, rp, fr, nw, tl, rt, sc, nx, A, stt); \
+    else                                                                                               \
+      sgd_fused_kernel<DD, NextT, false, TWIN><<<grid, 256, 0, s>>>(H, T, H0, T0, rp, fr, nw, tl, rt, sc, nx, A, stt); \
+    break;
+  switch (st->dim) {
+    UMAP_FUSED_CASE(1)
+    UMAP_FUSED_CASE(2)
+    UMAP_FUSED_CASE(3)
+    UMAP_FUSED_CASE(4)
+    UMAP_FUSED_CASE(5)
+    UMAP_FUSED_CASE(6)
+    UMAP_FUSED_CASE(7)
+    UMAP_FUSED_CASE(8)
+    default:
+      throw Error(Err::Unsupported, "row-owner SGD supports n_components <= 8");
+  }
+#undef UMAP_FUSED_CASE
+  UMAP_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -936,6 +1464,7 @@ void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t di
   }
   build_perm(st, s);
   alloc_edge_parallel(st, s);
+  setup_owner(st, s);
   st->stats = DevBuf(16);
   UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
   UMAP_HIP(hipEventRecord(ev1, s));
@@ -993,6 +1522,7 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
   UMAP_HIP(hipStreamSynchronize(s));
   build_perm(st, s);
   alloc_edge_parallel(st, s);
+  setup_owner(st, s);
   st->stats = DevBuf(16);
   UMAP_HIP(hipMemsetAsync(st->stats.p, 0, 16, s));
   UMAP_HIP(hipStreamSynchronize(s));

@@ -7,7 +7,14 @@
 
 namespace umapgpu {
 
-enum class SgdMode : int32_t { Hogwild = 0, Serial = 1, HogwildRacy = 2, HogwildRows = 3 };
+enum class SgdMode : int32_t {
+  Hogwild = 0,
+  Serial = 1,
+  HogwildRacy = 2,
+  HogwildRows = 3,
+  HogwildEdgeAtomic = 4,
+  HogwildWindowAtomic = 5
+};
 
 // Opt-in departures from the reference's SGD semantics (default 0 = the
 // reference).  kSemanticsSharedBuffer: head and tail updates share one array,
@@ -60,6 +67,16 @@ struct OptimizerState {
   DevBuf active;    // u32[E]: this epoch's active edges (edge-parallel mode)
   DevBuf counters;  // u32[2]: active-list lengths, alternating by epoch parity
   DevBuf stats;     // u64[2]: positive updates, negative samples
+
+  // Row-owner Hogwild (symmetric schedules).  twin_ok: every edge (j,k) has
+  // a twin (k,j) whose schedule state is bit-identical, so both are active in
+  // exactly the same epochs and the owner of row j can apply the twin's
+  // tail-side update to T[j] itself.
+  bool twin_ok = false;
+  uint64_t n_lblocks = 0;   // windows: whole rows whose first edge falls in [w*W, (w+1)*W)
+  uint64_t list_window = 0;  // W
+  DevBuf lblock_row;        // u32[n_lblocks+1]: first row of each window
+  DevBuf snap_h, snap_t;    // epoch-start copies of H and T (synchronous owner mode)
 
   double last_step_ms = 0.0;
   std::vector<StageTime> prep_timings;

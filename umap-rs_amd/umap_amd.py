@@ -34,7 +34,8 @@ P = C.c_void_p
 
 OK, INVALID_ARGUMENT, OUT_OF_MEMORY, HIP_ERROR, OVERSTEP, NO_DEVICE, UNSUPPORTED = range(7)
 
-SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY, SGD_HOGWILD_ROWS = 0, 1, 2, 3
+SGD_HOGWILD, SGD_SERIAL, SGD_HOGWILD_RACY, SGD_HOGWILD_ROWS, SGD_HOGWILD_EDGE_ATOMIC, SGD_HOGWILD_WINDOW_ATOMIC = (
+    0, 1, 2, 3, 4, 5)
 # opt-in SGD semantics bits (0 = the reference)
 SEMANTICS_SHARED_BUFFER, SEMANTICS_UMAP_LEARN_FORCES = 1, 2
 
