@@ -24,6 +24,9 @@
 // sequential CPU run bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "libm_glibc.hpp"
 #include "optimize.hpp"
 #include "primitives.hpp"
@@ -753,8 +756,11 @@ __global__ void twin_check_kernel(const uint64_t* __restrict__ row_ptr, const ui
 // written once, by its owner: no list, no counters, no atomics on the
 // embedding.  Windows are dealt to XCDs in eighths (block b runs on XCD
 // b % 8), so each XCD's L2 keeps its own rows from one epoch to the next.
-constexpr uint32_t kFusedPass = 1024;     // edges compacted per pass (4 a thread)
-constexpr uint32_t kFusedRowsLds = 2048;  // row pointers staged in LDS (else read from memory)
+constexpr uint32_t kFusedRowsLds = 1024;  // row pointers staged in LDS (else read from memory)
+#ifndef UMAP_SGD_BT
+#define UMAP_SGD_BT 256
+#endif
+constexpr int kFusedBlock = UMAP_SGD_BT;  // threads per row-owner block
 
 // Reads of another row and the owner's writes go through the device's
 // coherence point (agent-scope relaxed atomics: sc1 loads and stores), so a
@@ -829,263 +835,285 @@ __device__ inline void finish_row(float* H, float* T, uint32_t j, const float* h
   }
 }
 
-template <int D, typename NextT, bool SHARED, bool TWIN>
-__global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
-    float* H, float* T, const float* __restrict__ H0, const float* __restrict__ T0,
+template <int D, typename NextT, bool SHARED, bool TWIN, int BT>
+__global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
+    float* H, float* T, const float* __restrict__ HT0, const float* __restrict__ T0,
     const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ first_row,
     uint64_t n_windows, const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
     double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
+  constexpr int NW = BT / 64;         // waves per block
+  constexpr uint32_t PASS = 4 * BT;   // edges scanned per pass (4 a thread)
+  constexpr uint32_t RING = 8 * BT;   // queue of active edges (holds < BT + PASS)
+  constexpr uint32_t RP = kFusedRowsLds;
   __shared__ uint64_t s_exp2[32];
   __shared__ double s_log2[32];
-  __shared__ uint64_t s_ptr[kFusedRowsLds + 1];
-  __shared__ uint32_t s_e[kFusedPass], s_j[kFusedPass];
-  __shared__ float s_dh[256][D], s_hk[256][D];  // per lane: head change, twin tail change
-  __shared__ uint32_t s_cnt[4];
+  __shared__ uint64_t s_ptr[RP + 1];
+  __shared__ uint32_t s_e[RING], s_j[RING];
+  __shared__ float s_dh[BT][D], s_hk[BT][D];  // per lane: head change, twin tail change
+  __shared__ uint32_t s_cnt[NW];
   __shared__ RowCarry<D> s_carry[2];
   if (threadIdx.x < 32) {
     const LibmTables c = libm_default_tables();
     s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
     s_log2[threadIdx.x] = c.log2[threadIdx.x];
   }
-  // block -> window: XCD x = b % 8 takes the x-th eighth of the windows
-  const uint32_t x = blockIdx.x % kXcds, q8 = blockIdx.x / kXcds;
-  const uint64_t w_lo = (n_windows * x) / kXcds, w_hi = (n_windows * (x + 1)) / kXcds;
-  const uint64_t win = w_lo + q8;
-  if (win >= w_hi) return;  // block-uniform, before any barrier
-  const uint32_t r0 = first_row[win], r1 = first_row[win + 1];
-  const uint64_t e0 = row_ptr[r0], e1 = row_ptr[r1];
-  if (e0 == e1) return;  // block-uniform
-  const uint32_t nr = r1 - r0;
-  const bool ptr_lds = nr <= kFusedRowsLds;
-  if (ptr_lds)
-    for (uint32_t q = threadIdx.x; q <= nr; q += blockDim.x) s_ptr[q] = row_ptr[r0 + q];
-  if (threadIdx.x == 0) s_carry[0].j = s_carry[1].j = 0xffffffffu;
-  __syncthreads();
   const LibmTables tb{s_exp2, s_log2};
   const LaneFit F{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
   const NextT ep = (NextT)A.epoch;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint64_t npos = 0, nneg = 0;
-  uint32_t chunk_no = 0;
-  for (uint64_t pb = e0 & ~3ull; pb < e1; pb += kFusedPass) {  // block-uniform
-    // ---- compact this pass's active edges into s_e / s_j (edge order)
-    const uint64_t ef = pb + 4ull * threadIdx.x;
-    uint32_t mask = 0;
-    if (ef < e1 && ef + 4 > e0) {
-      NextT v[4];
-      if constexpr (sizeof(NextT) == 2) {
-        const unsigned long long qv = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(next + ef));
+  // blocks -> windows: XCD x = b % 8 walks the x-th eighth of the windows
+  const uint32_t x = blockIdx.x % kXcds;
+  const uint64_t per_xcd = gridDim.x / kXcds;
+  const uint64_t w_lo = (n_windows * x) / kXcds, w_hi = (n_windows * (x + 1)) / kXcds;
+  for (uint64_t win = w_lo + blockIdx.x / kXcds; win < w_hi; win += per_xcd) {  // block-uniform
+    const uint32_t r0 = first_row[win], r1 = first_row[win + 1];
+    const uint64_t e0 = row_ptr[r0], e1 = row_ptr[r1];
+    if (e0 == e1) continue;  // block-uniform
+    const uint32_t nr = r1 - r0;
+    const bool ptr_lds = nr <= RP;
+    __syncthreads();  // the previous window is done with s_ptr and the carry
+    if (ptr_lds)
+      for (uint32_t q = threadIdx.x; q <= nr; q += BT) s_ptr[q] = row_ptr[r0 + q];
+    if (threadIdx.x == 0) s_carry[0].j = s_carry[1].j = 0xffffffffu;
+    uint32_t q_head = 0, q_tail = 0, chunk_no = 0;  // block-uniform queue state
+    for (uint64_t pb = e0 & ~3ull;;) {  // block-uniform
+      const bool more = pb < e1;
+      if (more) {
+        // ---- append this pass's active edges to the queue (edge order)
+        const uint64_t ef = pb + 4ull * threadIdx.x;
+        uint32_t mask = 0;
+        if (ef < e1 && ef + 4 > e0) {
+          NextT v[4];
+          if constexpr (sizeof(NextT) == 2) {
+            const unsigned long long qv =
+                __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(next + ef));
 #pragma unroll
-        for (int t = 0; t < 4; ++t) v[t] = (NextT)((qv >> (16 * t)) & 0xffffu);
-      } else {
-        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-        const u32x4 qv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(next + ef));
-        v[0] = qv.x; v[1] = qv.y; v[2] = qv.z; v[3] = qv.w;
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const uint64_t e = ef + t;
-        if (e >= e0 && e < e1 && v[t] <= ep) mask |= 1u << t;
-      }
-    }
-    const uint32_t c = (uint32_t)__popc(mask);
-    uint32_t incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t u = __shfl_up(incl, (unsigned)off, 64);
-      if (lane >= off) incl += u;
-    }
-    if (lane == 63) s_cnt[w] = incl;
-    __syncthreads();
-    const uint32_t m = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    if (mask) {
-      uint32_t pos = incl - c;
-      for (int q = 0; q < w; ++q) pos += s_cnt[q];
-      const uint64_t efirst = ef + (uint64_t)__ffs(mask) - 1;
-      uint32_t lo = 0, hi = nr;  // the row q with ptr[q] <= efirst < ptr[q+1]
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint64_t pm = ptr_lds ? s_ptr[mid] : row_ptr[r0 + mid];
-        if (pm <= efirst) lo = mid; else hi = mid;
-      }
-      uint32_t q = lo;
-      uint64_t nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        if (!((mask >> t) & 1u)) continue;
-        const uint64_t e = ef + t;
-        while (nstart <= e) {
-          ++q;
-          nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
-        }
-        s_e[pos] = (uint32_t)e;
-        s_j[pos] = r0 + q;
-        ++pos;
-      }
-    }
-    __syncthreads();
-    // ---- run the active edges, 256 at a time
-    for (uint32_t cb = 0; cb < m; cb += 256, ++chunk_no) {  // block-uniform
-      const uint32_t pi = cb + threadIdx.x;
-      const bool valid = pi < m;
-      uint32_t j = 0xffffffffu;
-      float h0[D], t0[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) h0[d] = t0[d] = 0.0f;
-      if (valid) {
-        const uint32_t e = s_e[pi];
-        j = s_j[pi];
-        const uint32_t k = __builtin_nontemporal_load(tail + e);
-        double2 sc, rt;
-        sc.x = __builtin_nontemporal_load(&sched[e].x);
-        sc.y = __builtin_nontemporal_load(&sched[e].y);
-        rt.x = __builtin_nontemporal_load(&rate[e].x);
-        rt.y = __builtin_nontemporal_load(&rate[e].y);
-        sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
-        const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
-        sc.y = sc.y + (double)nn * rt.y;                             // :343
-        __builtin_nontemporal_store(sc.x, &sched[e].x);
-        __builtin_nontemporal_store(sc.y, &sched[e].y);
-        __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
-        const uint64_t ekey = rng_edge_key(F.epoch_key, e);
-        uint32_t self_mask = 0;
-        float tn[kPrefetch][D];
-#pragma unroll
-        for (int q = 0; q < kPrefetch; ++q) {
-          const uint32_t kq = rng_index(ekey, (uint64_t)q, F.n);
-          self_mask |= (kq == j) ? (1u << q) : 0u;
-          if ((uint32_t)q < nn) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) tn[q][d] = TWIN ? T0[(uint64_t)kq * D + d] : T[(uint64_t)kq * D + d];
-          }
-        }
-        float tk[D], hk[D], h[D], diff[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (TWIN) {  // every read from the epoch-start snapshot
-            h0[d] = H0[(uint64_t)j * D + d];
-            tk[d] = T0[(uint64_t)k * D + d];
-            t0[d] = T0[(uint64_t)j * D + d];
-            hk[d] = H0[(uint64_t)k * D + d];
+            for (int t = 0; t < 4; ++t) v[t] = (NextT)((qv >> (16 * t)) & 0xffffu);
           } else {
-            h0[d] = H[(uint64_t)j * D + d];
-            tk[d] = remote_load(T + (uint64_t)k * D + d);
-            hk[d] = 0.0f;
+            typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 qv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(next + ef));
+            v[0] = qv.x; v[1] = qv.y; v[2] = qv.z; v[3] = qv.w;
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const uint64_t e = ef + t;
+            if (e >= e0 && e < e1 && v[t] <= ep) mask |= 1u << t;
           }
         }
-        {  // own attractive sample (:278-306)
-          float d2 = 0.0f;
+        const uint32_t c = (uint32_t)__popc(mask);
+        uint32_t incl = c;
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            diff[d] = h0[d] - tk[d];
-            d2 = d2 + diff[d] * diff[d];
-          }
-          const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
-          float ga[D];
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            ga[d] = clip4(gc * diff[d]) * F.alpha;
-            h[d] = h0[d] + ga[d];
-          }
-          if (!TWIN) tail_sub<D>(T + (uint64_t)k * D, tk, ga);  // this edge's tail side, now
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t u = __shfl_up(incl, (unsigned)off, 64);
+          if (lane >= off) incl += u;
         }
-        for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
-          bool skip;
-          float o[D];
-          if (p < (uint32_t)kPrefetch) {
-            skip = (self_mask >> p) & 1u;
+        if (lane == 63) s_cnt[w] = incl;
+        __syncthreads();  // (also orders s_ptr staging before its first use)
+        uint32_t m = 0, before = 0;
 #pragma unroll
-            for (int q = 0; q < kPrefetch; ++q)
-              if ((uint32_t)q == p) {
+        for (int q = 0; q < NW; ++q) {
+          m += s_cnt[q];
+          before += (q < w) ? s_cnt[q] : 0u;
+        }
+        if (mask) {
+          uint32_t pos = q_tail + before + incl - c;
+          const uint64_t efirst = ef + (uint64_t)__ffs(mask) - 1;
+          uint32_t lo = 0, hi = nr;  // the row q with ptr[q] <= efirst < ptr[q+1]
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const uint64_t pm = ptr_lds ? s_ptr[mid] : row_ptr[r0 + mid];
+            if (pm <= efirst) lo = mid; else hi = mid;
+          }
+          uint32_t q = lo;
+          uint64_t nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
 #pragma unroll
-                for (int d = 0; d < D; ++d) o[d] = tn[q][d];
-              }
+          for (int t = 0; t < 4; ++t) {
+            if (!((mask >> t) & 1u)) continue;
+            const uint64_t e = ef + t;
+            while (nstart <= e) {
+              ++q;
+              nstart = ptr_lds ? s_ptr[q + 1] : row_ptr[r0 + q + 1];
+            }
+            s_e[pos & (RING - 1)] = (uint32_t)e;
+            s_j[pos & (RING - 1)] = r0 + q;
+            ++pos;
+          }
+        }
+        q_tail += m;
+        pb += PASS;
+        __syncthreads();
+      }
+      // ---- run full chunks (and, once the window is scanned, the rest)
+      while (q_tail - q_head >= (uint32_t)BT || (!more && q_tail > q_head)) {  // block-uniform
+        const uint32_t cnt = min(q_tail - q_head, (uint32_t)BT);
+        const bool valid = threadIdx.x < cnt;
+        const uint32_t slot = (q_head + threadIdx.x) & (RING - 1);
+        uint32_t j = 0xffffffffu;
+        float h0[D], t0[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) h0[d] = t0[d] = 0.0f;
+        if (valid) {
+          const uint32_t e = s_e[slot];
+          j = s_j[slot];
+          const uint32_t k = __builtin_nontemporal_load(tail + e);
+          double2 sc, rt;
+          sc.x = __builtin_nontemporal_load(&sched[e].x);
+          sc.y = __builtin_nontemporal_load(&sched[e].y);
+          rt.x = __builtin_nontemporal_load(&rate[e].x);
+          rt.y = __builtin_nontemporal_load(&rate[e].y);
+          sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
+          const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
+          sc.y = sc.y + (double)nn * rt.y;                             // :343
+          __builtin_nontemporal_store(sc.x, &sched[e].x);
+          __builtin_nontemporal_store(sc.y, &sched[e].y);
+          __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
+          const uint64_t ekey = rng_edge_key(F.epoch_key, e);
+          uint32_t self_mask = 0;
+          float tn[kPrefetch][D];
+#pragma unroll
+          for (int q = 0; q < kPrefetch; ++q) {
+            const uint32_t kq = rng_index(ekey, (uint64_t)q, F.n);
+            self_mask |= (kq == j) ? (1u << q) : 0u;
+            if ((uint32_t)q < nn) {
+#pragma unroll
+              for (int d = 0; d < D; ++d) tn[q][d] = TWIN ? T0[(uint64_t)kq * D + d] : T[(uint64_t)kq * D + d];
+            }
+          }
+          float tk[D], hk[D], h[D], diff[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            if (TWIN && !SHARED) {  // every read from the epoch-start copy; HT0 rows are {H, T}
+              h0[d] = HT0[(uint64_t)j * 2 * D + d];
+              t0[d] = HT0[(uint64_t)j * 2 * D + D + d];
+              hk[d] = HT0[(uint64_t)k * 2 * D + d];
+              tk[d] = HT0[(uint64_t)k * 2 * D + D + d];
+            } else if (TWIN) {  // one shared array: its copy is T0
+              h0[d] = t0[d] = T0[(uint64_t)j * D + d];
+              hk[d] = tk[d] = T0[(uint64_t)k * D + d];
+            } else {
+              h0[d] = H[(uint64_t)j * D + d];
+              tk[d] = remote_load(T + (uint64_t)k * D + d);
+              hk[d] = 0.0f;
+            }
+          }
+          {  // own attractive sample (:278-306)
+            float d2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              diff[d] = h0[d] - tk[d];
+              d2 = d2 + diff[d] * diff[d];
+            }
+            const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+            float ga[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              ga[d] = clip4(gc * diff[d]) * F.alpha;
+              h[d] = h0[d] + ga[d];
+            }
+            if (!TWIN) tail_sub<D>(T + (uint64_t)k * D, tk, ga);  // this edge's tail side, now
+          }
+          for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
+            bool skip;
+            float o[D];
+            if (p < (uint32_t)kPrefetch) {
+              skip = (self_mask >> p) & 1u;
+#pragma unroll
+              for (int q = 0; q < kPrefetch; ++q)
+                if ((uint32_t)q == p) {
+#pragma unroll
+                  for (int d = 0; d < D; ++d) o[d] = tn[q][d];
+                }
+            } else {
+              const uint32_t kp = rng_index(ekey, p, F.n);
+              skip = (kp == j);
+#pragma unroll
+              for (int d = 0; d < D; ++d) o[d] = TWIN ? T0[(uint64_t)kp * D + d] : T[(uint64_t)kp * D + d];
+            }
+            if (skip) continue;
+            float d2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              diff[d] = h[d] - o[d];
+              d2 = d2 + diff[d] * diff[d];
+            }
+            const float g2 = (d2 > 0.0f) ? repel_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+            if (g2 > 0.0f) {
+#pragma unroll
+              for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * F.alpha;
+            }
+          }
+          float dt[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) dt[d] = 0.0f;
+          if (TWIN) {  // the twin edge (k,j)'s tail side, from the same values its head side uses
+            float d2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              diff[d] = hk[d] - t0[d];
+              d2 = d2 + diff[d] * diff[d];
+            }
+            const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * F.alpha);
+          }
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            s_dh[threadIdx.x][d] = h[d] - h0[d];
+            s_hk[threadIdx.x][d] = dt[d];
+          }
+          ++npos;
+          nneg += nn;
+        }
+        __syncthreads();
+        // the chunk's last entry holds the run that may go on in the next
+        // chunk: it becomes the carry instead of being written
+        const uint32_t last = cnt - 1;
+        const uint32_t par = chunk_no & 1u;
+        const RowCarry<D>& cin = s_carry[par];
+        if (threadIdx.x == 0 && cin.j != 0xffffffffu && cin.j != j)  // the carry ends before this chunk
+          finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
+        if (valid && (threadIdx.x == last || s_j[(slot + 1) & (RING - 1)] != j)) {
+          uint32_t p = threadIdx.x;  // the run's first position in this chunk
+          while (p > 0 && s_j[(q_head + p - 1) & (RING - 1)] == j) --p;
+          const bool cont = (p == 0) && (cin.j == j);
+          float dh[D], t[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dh[d] = cont ? cin.dh[d] : 0.0f;
+            t[d] = cont ? cin.t[d] : t0[d];
+          }
+          for (uint32_t q = p; q <= threadIdx.x; ++q) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              dh[d] = dh[d] + s_dh[q][d];
+              t[d] = t[d] + s_hk[q][d];
+            }
+          }
+          if (threadIdx.x == last) {
+            RowCarry<D>& co = s_carry[par ^ 1u];
+            co.j = j;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              co.h0[d] = h0[d];
+              co.t0[d] = t0[d];
+              co.dh[d] = dh[d];
+              co.t[d] = t[d];
+            }
           } else {
-            const uint32_t kp = rng_index(ekey, p, F.n);
-            skip = (kp == j);
-#pragma unroll
-            for (int d = 0; d < D; ++d) o[d] = TWIN ? T0[(uint64_t)kp * D + d] : T[(uint64_t)kp * D + d];
-          }
-          if (skip) continue;
-          float d2 = 0.0f;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            diff[d] = h[d] - o[d];
-            d2 = d2 + diff[d] * diff[d];
-          }
-          const float g2 = (d2 > 0.0f) ? repel_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
-          if (g2 > 0.0f) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * F.alpha;
+            finish_row<D, SHARED, TWIN>(H, T, j, h0, dh, t0, t);
           }
         }
-        float dt[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) dt[d] = 0.0f;
-        if (TWIN) {  // the twin edge (k,j)'s tail side, from the same values its head side uses
-          float d2 = 0.0f;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            diff[d] = hk[d] - t0[d];
-            d2 = d2 + diff[d] * diff[d];
-          }
-          const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
-#pragma unroll
-          for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * F.alpha);
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          s_dh[threadIdx.x][d] = h[d] - h0[d];
-          s_hk[threadIdx.x][d] = dt[d];
-        }
-        ++npos;
-        nneg += nn;
+        q_head += cnt;
+        ++chunk_no;
+        __syncthreads();  // s_dh / s_hk / the carry are rewritten by the next chunk
       }
-      __syncthreads();
-      // the chunk's last position holds the run that may go on in the next
-      // chunk: it becomes the carry instead of being written
-      const uint32_t last = (m - cb >= 256) ? 255u : (m - cb - 1);
-      const uint32_t par = chunk_no & 1u;
-      const RowCarry<D>& cin = s_carry[par];
-      if (threadIdx.x == 0 && cin.j != 0xffffffffu && cin.j != j)  // the carry ends before this chunk
-        finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
-      if (valid && (threadIdx.x == last || s_j[pi + 1] != j)) {
-        uint32_t p = threadIdx.x;  // the run's first position in this chunk
-        while (p > 0 && s_j[cb + p - 1] == j) --p;
-        const bool cont = (p == 0) && (cin.j == j);
-        float dh[D], t[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dh[d] = cont ? cin.dh[d] : 0.0f;
-          t[d] = cont ? cin.t[d] : t0[d];
-        }
-        for (uint32_t q = p; q <= threadIdx.x; ++q) {
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            dh[d] = dh[d] + s_dh[q][d];
-            t[d] = t[d] + s_hk[q][d];
-          }
-        }
-        if (threadIdx.x == last) {
-          RowCarry<D>& co = s_carry[par ^ 1u];
-          co.j = j;
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            co.h0[d] = h0[d];
-            co.t0[d] = t0[d];
-            co.dh[d] = dh[d];
-            co.t[d] = t[d];
-          }
-        } else {
-          finish_row<D, SHARED, TWIN>(H, T, j, h0, dh, t0, t);
-        }
-      }
-      __syncthreads();  // s_dh / s_hk / the carry are rewritten by the next chunk
+      if (!more) break;
     }
-  }
-  if (threadIdx.x == 0) {  // the last pending run
-    const RowCarry<D>& cin = s_carry[chunk_no & 1u];
-    if (cin.j != 0xffffffffu) finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
+    if (threadIdx.x == 0) {  // the window's last pending run
+      const RowCarry<D>& cin = s_carry[chunk_no & 1u];
+      if (cin.j != 0xffffffffu) finish_row<D, SHARED, TWIN>(H, T, cin.j, cin.h0, cin.dh, cin.t0, cin.t);
+    }
   }
   wave_add_stats(npos, nneg, stats);
 }
@@ -1100,6 +1128,21 @@ __global__ void add_u64_kernel(uint64_t* __restrict__ dst, const uint64_t* __res
                                uint64_t add) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i] + add;
+}
+
+// Epoch-start copy for the row-owner epoch: HT0 rows {H[r], T[r]} side by
+// side (a row's head and tail values share one line), T0 a plain copy of T
+// (the negative samples read only tails).
+__global__ void snapshot_kernel(float* __restrict__ HT0, float* __restrict__ T0, const float* __restrict__ H,
+                                const float* __restrict__ T, uint64_t n, uint32_t dim) {
+  const uint64_t count = n * dim;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / dim, d = i - r * dim;
+    const float t = T[i];
+    HT0[r * 2 * dim + d] = H[i];
+    HT0[r * 2 * dim + dim + d] = t;
+    T0[i] = t;
+  }
 }
 
 // Plain copy as a kernel, so that it can sit inside a captured graph.
@@ -1287,9 +1330,12 @@ void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
 // over more chunks; measured at 1M points: 2048 edges 1339 us/epoch, 4096
 // 938, 8192 865.  Small graphs use smaller windows so that there are still
 // about 4096 blocks.
+#ifndef UMAP_WIN_MAX
+#define UMAP_WIN_MAX 8192
+#endif
 uint64_t owner_window(uint64_t n_edges) {
   uint64_t w = 256;
-  while (w < 8192 && w * 4096 < n_edges) w <<= 1;
+  while (w < UMAP_WIN_MAX && w * 4096 < n_edges) w <<= 1;
   return w;
 }
 
@@ -1321,12 +1367,37 @@ void setup_owner(OptimizerState* st, hipStream_t s) {
   }
 }
 
+// Persistent grid for the row-owner kernel: as many blocks as stay resident
+// (a multiple of 8, one share per XCD), never more than there are windows.
+int resident_blocks(const void* kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;  // (kernel, device) -> blocks
+  int dev = 0;
+  UMAP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({kernel, dev});
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  UMAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  UMAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+  const int r = std::max(1, per_cu) * std::max(1, cus);
+  cache[{kernel, dev}] = r;
+  return r;
+}
+
+template <typename K, typename... Args>
+void launch_fused(K kernel, uint64_t n_windows, hipStream_t s, Args... args) {
+  const int resident = resident_blocks(reinterpret_cast<const void*>(kernel), kFusedBlock);
+  const uint64_t want = std::min<uint64_t>(n_windows, (uint64_t)resident);
+  const unsigned grid = (unsigned)(kXcds * ceil_div(std::max<uint64_t>(want, 1), kXcds));
+  kernel<<<grid, kFusedBlock, 0, s>>>(args...);
+}
+
 template <typename NextT, bool TWIN>
 void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   float* H = st->head_emb.as<float>();
   float* T = tail_buffer(st);
   const bool shared = (st->semantics & kSemanticsSharedBuffer) != 0;
-  const unsigned grid = (unsigned)(kXcds * ceil_div(st->n_lblocks, kXcds));
   const uint64_t* rp = st->row_ptr.as<uint64_t>();
   const uint32_t* fr = st->lblock_row.as<uint32_t>();
   const uint32_t* tl = st->tail.as<uint32_t>();
@@ -1335,28 +1406,31 @@ void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   NextT* nx = st->next.as<NextT>();
   unsigned long long* stt = st->stats.as<unsigned long long>();
   const uint64_t nw = st->n_lblocks;
-  const float* H0 = nullptr;
+  const float* HT0 = nullptr;
   const float* T0 = nullptr;
-  if (TWIN) {  // epoch-start snapshots of both arrays
+  if (TWIN) {  // epoch-start copies
     const uint64_t nel = st->n * st->dim;
-    if (st->snap_h.bytes < nel * 4) st->snap_h = DevBuf(nel * 4);
-    copy_f32_kernel<<<(unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096), 256, 0, s>>>(st->snap_h.as<float>(), H, nel);
-    UMAP_LAUNCH_CHECK();
-    H0 = st->snap_h.as<float>();
-    T0 = H0;
-    if (!shared) {
-      if (st->snap_t.bytes < nel * 4) st->snap_t = DevBuf(nel * 4);
-      copy_f32_kernel<<<(unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096), 256, 0, s>>>(st->snap_t.as<float>(), T, nel);
-      UMAP_LAUNCH_CHECK();
-      T0 = st->snap_t.as<float>();
+    const unsigned cgrid = (unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096);
+    if (st->snap_t.bytes < nel * 4) st->snap_t = DevBuf(nel * 4);
+    if (shared) {
+      copy_f32_kernel<<<cgrid, 256, 0, s>>>(st->snap_t.as<float>(), H, nel);
+    } else {
+      if (st->snap_h.bytes < 2 * nel * 4) st->snap_h = DevBuf(2 * nel * 4);
+      snapshot_kernel<<<cgrid, 256, 0, s>>>(st->snap_h.as<float>(), st->snap_t.as<float>(), H, T, st->n,
+                                            (uint32_t)st->dim);
+      HT0 = st->snap_h.as<float>();
     }
+    UMAP_LAUNCH_CHECK();
+    T0 = st->snap_t.as<float>();
   }
-#define UMAP_FUSED_CASE(DD)                                                                            \
-  case DD:                                                                                             \
-    if (shared)                                                                                        \
-      sgd_fused_kernel<DD, NextT, true, TWIN><<<grid, 256, 0, s>>>(H, T, H0, T0, rp, fr, nw, tl, rt, sc, nx, A, stt); \
-    else                                                                                               \
-      sgd_fused_kernel<DD, NextT, false, TWIN><<<grid, 256, 0, s>>>(H, T, H0, T0, rp, fr, nw, tl, rt, sc, nx, A, stt); \
+#define UMAP_FUSED_CASE(DD)                                                                  \
+  case DD:                                                                                   \
+    if (shared)                                                                              \
+      launch_fused(sgd_fused_kernel<DD, NextT, true, TWIN, kFusedBlock>, nw, s, H, T, HT0, T0, rp, fr, nw, tl, rt, sc, \
+                   nx, A, stt);                                                              \
+    else                                                                                     \
+      launch_fused(sgd_fused_kernel<DD, NextT, false, TWIN, kFusedBlock>, nw, s, H, T, HT0, T0, rp, fr, nw, tl, rt, sc, \
+                   nx, A, stt);                                                              \
     break;
   switch (st->dim) {
     UMAP_FUSED_CASE(1)
@@ -1414,7 +1488,7 @@ void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t di
   UMAP_HIP(hipMemcpyAsync(&st->n_edges, st->row_ptr.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
   UMAP_HIP(hipStreamSynchronize(s));
   const uint64_t E = st->n_edges;
-  st->tail = DevBuf(std::max<uint64_t>(E, 1) * 4);
+  st->tail = DevBuf((ceil_div(E, 4) * 4 + 4) * 4);  // padded: the row-owner scan reads 4 tails at a time
   DevBuf w(std::max<uint64_t>(E, 1) * 4);
   if (n) {
     filter_fill_kernel<<<grid_for(n, 256), 256, 0, s>>>(g.indptr.as<uint64_t>(), g.indices.as<uint32_t>(),
@@ -1499,7 +1573,7 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
   }
   const uint64_t E1 = std::max<uint64_t>(n_edges, 1);
   st->row_ptr = DevBuf((n + 1) * 8);
-  st->tail = DevBuf(E1 * 4);
+  st->tail = DevBuf((ceil_div(n_edges, 4) * 4 + 4) * 4);  // padded as in optimizer_prepare
   st->rate = DevBuf(E1 * 16);
   st->sched = DevBuf(E1 * 16);
   alloc_next(st);

@@ -76,7 +76,7 @@ struct OptimizerState {
   uint64_t n_lblocks = 0;   // windows: whole rows whose first edge falls in [w*W, (w+1)*W)
   uint64_t list_window = 0;  // W
   DevBuf lblock_row;        // u32[n_lblocks+1]: first row of each window
-  DevBuf snap_h, snap_t;    // epoch-start copies of H and T (synchronous owner mode)
+  DevBuf snap_h, snap_t;    // epoch-start copies: rows {H, T} side by side, and T alone
 
   double last_step_ms = 0.0;
   std::vector<StageTime> prep_timings;
