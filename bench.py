@@ -450,7 +450,7 @@ def large_graph_section(U, args):
     log(f"large graph: Optimizer::new {new_s:.2f}s, SGD {sgd_s:.2f}s ({ms:.1f} ms device), {p:,} updates, "
         f"{rate:.4g} updates/s")
     return {"workload": f"n={n:,} D=64: 200-component mixture + 5% uniform background, kNN k={k}, symmetrised, "
-                        f"2-D, min_dist 0.05, {epochs} epochs, Hogwild",
+                        f"2-D, min_dist 0.05, {epochs} epochs, parallel SGD (default mode)",
             "knn_s": round(knn_s, 2), "knn_mixture_rows_certificate": round(worst, 5),
             "knn_note": "mixture rows exact by component when the certificate is < 1; background rows' neighbours "
                         "are taken among the background only",
@@ -472,7 +472,8 @@ def main() -> int:
     ap.add_argument("--dim", type=int, default=64)
     ap.add_argument("--k", type=int, default=15)
     ap.add_argument("--epochs", type=int, default=500)
-    ap.add_argument("--mode", choices=["hogwild", "hogwild_racy", "hogwild_rows"], default="hogwild")
+    ap.add_argument("--mode", choices=["hogwild", "hogwild_window_atomic", "hogwild_edge_atomic", "hogwild_racy",
+                                       "hogwild_rows"], default="hogwild")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's SGD sample")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-quality", action="store_true")
@@ -544,7 +545,8 @@ def main() -> int:
         src.free()
         dst.free()
 
-    mode = {"hogwild": U.SGD_HOGWILD, "hogwild_racy": U.SGD_HOGWILD_RACY,
+    mode = {"hogwild": U.SGD_HOGWILD, "hogwild_window_atomic": U.SGD_HOGWILD_WINDOW_ATOMIC,
+            "hogwild_edge_atomic": U.SGD_HOGWILD_EDGE_ATOMIC, "hogwild_racy": U.SGD_HOGWILD_RACY,
             "hogwild_rows": U.SGD_HOGWILD_ROWS}[args.mode]
     times, kernel_ms, updates, negs, prep_ms = [], [], [], [], []
     last_opt = None
@@ -584,16 +586,19 @@ def main() -> int:
     achieved = bpu * float(sum(updates)) / kernel_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": {"hogwild": "sgd_edges_kernel<2,u16,atomic tail>",
-                           "hogwild_racy": "sgd_edges_kernel<2,u16,racy tail>",
+                "kernel": {"hogwild": "sgd_fused_kernel<2,u16,row-owner synchronous> (+ snapshot_kernel per epoch)",
+                           "hogwild_window_atomic": "sgd_fused_kernel<2,u16,per-edge atomic tail>",
+                           "hogwild_edge_atomic": "active_edges_kernel + sgd_edges_kernel<2,u16,atomic tail>",
+                           "hogwild_racy": "active_edges_kernel + sgd_edges_kernel<2,u16,racy tail>",
                            "hogwild_rows": "sgd_rows_kernel<2,u16,atomic tail>"}[args.mode],
                 "bytes_per_update": bpu, "avg_launch_us": round(1e6 * kernel_s / (epochs * len(kernel_ms)), 2),
                 "updates_per_launch": round(float(sum(updates)) / (epochs * len(kernel_ms)), 1)}
     traffic_file = os.path.join(REPO, "profiles", "sgd_traffic_c2.json")
-    if os.path.exists(traffic_file):
+    if os.path.exists(traffic_file) and args.mode == "hogwild" and (n, k, epochs) == (1_000_000, 15, 500):
         with open(traffic_file) as f:
             tr = json.load(f)
         roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+        roofline["traffic_note"] = tr.get("note")
 
     out = {
         "metric": "SGD edge-updates/s + wall-to-target-loss, learned-init vs plain, 1 MI355X",
@@ -609,7 +614,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture; exact kNN computed on the GPU)",
         "config": {"workload": f"plain UMAP fit path: n={n:,} D={dim} 50-component GMM, exact kNN k={k}, "
-                               f"2-D, min_dist 0.1, {epochs} epochs, Hogwild SGD",
+                               f"2-D, min_dist 0.1, {epochs} epochs, parallel SGD ({args.mode})",
                    "n": n, "D": dim, "k": k, "epochs": epochs, "sgd_mode": args.mode,
                    "edges_after_filter": int(n_edges), "graph_nnz": int(gt["nnz"])},
         "roofline": roofline,

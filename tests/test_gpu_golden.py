@@ -1,0 +1,66 @@
+"""Device results against the committed golden fixtures (tests/golden/*.npz,
+written by the CPU restatement; see tests/golden/make_golden.py).  Bar: bit
+for bit -- graph stage (sigma, rho, CSR), Optimizer::new state, five serial
+epochs and five default (row-owner synchronous) epochs."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view({4: np.uint32, 8: np.uint64}[a.dtype.itemsize])
+
+
+def _graph(U, z, k, **graph_kw):
+    cfg = U.UmapConfig()
+    cfg.graph.n_neighbors = k
+    for key, v in graph_kw.items():
+        setattr(cfg.graph, key, v)
+    n = z["knn_indices"].shape[0]
+    m = U.Umap(cfg).learn_manifold(np.zeros((n, 1), np.float32), z["knn_indices"], z["knn_dists"])
+    s, r = m.sigmas_rhos()
+    g = m.graph()
+    assert np.array_equal(_bits(s), _bits(z["sigmas"])) and np.array_equal(_bits(r), _bits(z["rhos"]))
+    assert np.array_equal(g.indptr(), z["indptr"]) and np.array_equal(g.indices(), z["indices"])
+    assert np.array_equal(_bits(g.data()), _bits(z["data"]))
+    return cfg, m
+
+
+def test_graph_gmm300(gpu):
+    z = np.load(os.path.join(GOLD, "graph_gmm300.npz"))
+    _, m = _graph(gpu, z, 10)
+    assert m.curve_params() == (float(z["a"]), float(z["b"]))
+
+
+def test_graph_ring50_mix_and_local_connectivity(gpu):
+    z = np.load(os.path.join(GOLD, "graph_ring50.npz"))
+    _graph(gpu, z, 10, set_op_mix_ratio=0.5, local_connectivity=2.0)
+
+
+@pytest.mark.parametrize("mode", ["serial", "default"])
+def test_sgd_gmm300(gpu, mode):
+    zg = np.load(os.path.join(GOLD, "graph_gmm300.npz"))
+    z = np.load(os.path.join(GOLD, "sgd_gmm300.npz"))
+    cfg, m = _graph(gpu, zg, 10)
+    opt = gpu.Optimizer.new(m, z["init"], int(z["total_epochs"]), cfg, gpu.MetricType.Euclidean, seed=int(z["seed"]))
+    st = opt.export_state()
+    assert np.array_equal(st["head"], z["head"]) and np.array_equal(st["tail"], z["tail"])
+    assert np.array_equal(_bits(st["epochs_per_sample"]), _bits(z["eps"]))
+    assert np.array_equal(_bits(st["epochs_per_negative_sample"]), _bits(z["epns"]))
+    assert np.array_equal(_bits(opt.embedding()), _bits(z["normalised"]))
+    if mode == "serial":
+        opt.set_mode(gpu.SGD_SERIAL)
+    opt.step_epochs(5)
+    key = "serial5" if mode == "serial" else "sync5"
+    assert np.array_equal(_bits(opt.embedding()), _bits(z[key]))
+    p, q, _ = opt.stats()
+    assert [p, q] == z[key + "_counts"].tolist()
+    if mode == "default":
+        st = opt.export_state()
+        assert np.array_equal(_bits(st["epoch_of_next_sample"]), _bits(z["sync5_eons"]))
+        assert np.array_equal(_bits(st["epoch_of_next_negative_sample"]), _bits(z["sync5_eonns"]))

@@ -1174,6 +1174,7 @@ template <typename NextT, bool ATOMIC_TAIL>
 void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   const uint32_t slot = (uint32_t)(A.epoch & 1u);
   const uint64_t cap = list_capacity(st->n_edges);
+  if (!st->active.p) st->active = DevBuf(cap * kShards * 4);
   active_edges_kernel<NextT><<<grid_for(st->n_edges, 256), 256, 0, s>>>(
       st->next.as<NextT>(), st->n_edges, (NextT)A.epoch, st->active.as<uint32_t>(), st->counters.as<uint32_t>(), slot,
       cap);
@@ -1316,7 +1317,7 @@ void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
   const uint64_t E = st->n_edges;
   if (E >= 0xffffffffull) throw Error(Err::Unsupported, "more than 2^32-1 SGD edges is not supported");
   st->head = DevBuf(std::max<uint64_t>(E, 1) * 4);
-  st->active = DevBuf(list_capacity(E) * kShards * 4);
+  st->active = DevBuf();  // allocated by the first list-kernel epoch
   st->counters = DevBuf(2 * kShards * kCounterStride * 4);
   UMAP_HIP(hipMemsetAsync(st->counters.p, 0, st->counters.bytes, s));
   if (st->n) {
