@@ -362,8 +362,8 @@ def sweep_section(U, args):
     sweep_s = time.perf_counter() - t
     st = sw.stats()
     info = sw.info()
-    log(f"sweep: one at a time {seq_s:.3f}s, batched {sweep_s:.3f}s (graph, grid {st['blocks_per_region']} blocks "
-        f"per region), updates {st['positive']}")
+    log(f"sweep: one at a time {seq_s:.3f}s, batched {sweep_s:.3f}s (graph, {st['edges_per_window']} edges per "
+        f"window), updates {st['positive']}")
     res = {"workload": f"{len(fits)} fits, n log-uniform in [5k, 500k] (total {total_n:,}), D=32 mixtures of 5-50 "
                        f"components, k in {{10,15,30}}, min_dist in {{0,0.1,0.5}}, 2-D, 300 epochs each",
            "rows": info["rows"], "edges": info["edges"], "inputs_and_graphs_s": round(prep_s, 2),
@@ -371,7 +371,7 @@ def sweep_section(U, args):
            "batched_edge_updates_per_s": round(st["positive"] / sweep_s, 1),
            "one_at_a_time_edge_updates_per_s": round(seq_updates / seq_s, 1),
            "speedup": round(seq_s / sweep_s, 3), "updates_equal": bool(seq_updates == st["positive"]),
-           "autotune": sw.tuning(), "blocks_per_region": st["blocks_per_region"]}
+           "autotune": sw.tuning(), "edges_per_window": st["edges_per_window"]}
     del sw, opts, prepared
     return res
 

@@ -316,12 +316,13 @@ UMAPGPU_API umapgpu_status umapgpu_sweep_set_graph(umapgpu_sweep* s, int32_t use
 UMAPGPU_API umapgpu_status umapgpu_sweep_info(const umapgpu_sweep* s, uint64_t* fits, uint64_t* rows, uint64_t* edges,
                                               uint64_t* current_epoch);
 /* Positive updates and negative samples over all calls; the last call's
- * device time; the chosen SGD blocks per region (grid = 128 x that). */
+ * device time; the edges per window the autotuner chose (each block of the
+ * parallel epoch walks windows of about that many edges, whole rows). */
 UMAPGPU_API umapgpu_status umapgpu_sweep_stats(const umapgpu_sweep* s, uint64_t* positive, uint64_t* negative,
-                                               double* last_step_ms, uint32_t* blocks_per_region);
-/* Autotuner record as JSON ([{"blocks_per_region": b, "ms": t, "updates": u},
- * ...]: each candidate's epoch time and positive updates; the grid with the
- * least time per update wins); same length convention as
+                                               double* last_step_ms, uint32_t* edges_per_window);
+/* Autotuner record as JSON ([{"edges_per_window": w, "ms": t, "updates": u},
+ * ...]: each candidate's epoch time and positive updates; the window size
+ * with the least time per update wins); same length convention as
  * umapgpu_manifold_timings_json. */
 UMAPGPU_API uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t len);
 UMAPGPU_API void umapgpu_sweep_destroy(umapgpu_sweep* s);

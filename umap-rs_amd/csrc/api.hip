@@ -957,7 +957,7 @@ umapgpu_status umapgpu_sweep_stats(const umapgpu_sweep* s, uint64_t* pos, uint64
     if (pos) *pos = p;
     if (neg) *neg = q;
     if (ms) *ms = s->sw.last_step_ms;
-    if (bps) *bps = s->sw.blocks_per_shard;
+    if (bps) *bps = s->sw.windows.empty() ? 0u : (uint32_t)s->sw.windows[s->sw.window_idx].size;
   });
 }
 
@@ -967,7 +967,7 @@ uint64_t umapgpu_sweep_tuning_json(const umapgpu_sweep* s, char* buf, uint64_t l
   for (size_t i = 0; i < s->sw.tuning.size(); ++i) {
     if (i) j += ",";
     const auto& t = s->sw.tuning[i];
-    j += "{\"blocks_per_region\":" + std::to_string(t.blocks_per_shard) + ",\"ms\":" + std::to_string(t.ms) +
+    j += "{\"edges_per_window\":" + std::to_string(t.window) + ",\"ms\":" + std::to_string(t.ms) +
          ",\"updates\":" + std::to_string(t.updates) + "}";
   }
   j += "]";

@@ -484,34 +484,22 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
 // the first kPrefetch negative targets, whose indices depend only on the
 // counter RNG -- is issued before any of them is used, so the lane waits
 // for one memory round trip instead of one per sample.
-//
-// SWEEP: the edge array is the concatenation of independent fits (a batched
-// sweep).  Each lane finds its edge's fit by binary search over the fits'
-// edge offsets in LDS and takes that fit's a, b, alpha, vertex count, row
-// offset and RNG key; negative samples stay inside the fit's own rows and
-// use the fit-local edge index, so every fit draws what it would alone.
-template <int D, typename NextT, bool ATOMIC_TAIL, bool SWEEP>
+template <int D, typename NextT, bool ATOMIC_TAIL>
 __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
                                                         const uint32_t* __restrict__ head,
                                                         const uint32_t* __restrict__ tail,
                                                         const double2* __restrict__ rate, double2* __restrict__ sched,
                                                         NextT* __restrict__ next, const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ counters, uint32_t slot,
-                                                        uint64_t cap, SgdArgs A, const SweepFit* __restrict__ fits,
-                                                        const SweepEpoch* __restrict__ fit_epoch, uint32_t nfits,
+                                                        uint64_t cap, SgdArgs A,
                                                         unsigned long long* __restrict__ stats) {
   // powf's two lookup tables, staged in LDS for low-latency divergent reads
   __shared__ uint64_t s_exp2[32];
   __shared__ double s_log2[32];
-  __shared__ uint64_t s_eoff[SWEEP ? kMaxSweepFits + 1 : 1];
   if (UMAP_SGD_LDS_TABLES && threadIdx.x < 32) {
     const LibmTables c = libm_default_tables();
     s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
     s_log2[threadIdx.x] = c.log2[threadIdx.x];
-  }
-  if constexpr (SWEEP) {
-    for (uint32_t f = threadIdx.x; f <= nfits; f += blockDim.x)
-      s_eoff[f] = f < nfits ? fits[f].edge_off : ~0ull;
   }
   __syncthreads();
   const LibmTables tb = UMAP_SGD_LDS_TABLES ? LibmTables{s_exp2, s_log2} : libm_default_tables();
@@ -535,22 +523,8 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
       const uint32_t e = __builtin_nontemporal_load(lst + i);
       j = __builtin_nontemporal_load(head + e);
       const uint32_t k = __builtin_nontemporal_load(tail + e);
-      // this edge's fit: the whole problem, or one member of a sweep
-      LaneFit F;
-      uint64_t e_local = e;
-      if constexpr (SWEEP) {
-        uint32_t lo = 0, hi = nfits;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_eoff[mid] <= (uint64_t)e) lo = mid; else hi = mid;
-        }
-        const SweepFit fp = fits[lo];
-        const SweepEpoch fe = fit_epoch[lo];
-        F = LaneFit{fp.a, fp.b, fp.m2ab, fp.g2b, fe.alpha, fp.n, fp.ref_den, fp.row_off, fe.epoch_key};
-        e_local = e - fp.edge_off;
-      } else {
-        F = LaneFit{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
-      }
+      const LaneFit F{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
+      const uint64_t e_local = e;
       double2 sc, rt;
       sc.x = __builtin_nontemporal_load(&sched[e].x);
       sc.y = __builtin_nontemporal_load(&sched[e].y);
@@ -835,12 +809,13 @@ __device__ inline void finish_row(float* H, float* T, uint32_t j, const float* h
   }
 }
 
-template <int D, typename NextT, bool SHARED, bool TWIN, int BT>
+template <int D, typename NextT, bool SHARED, bool TWIN, int BT, bool SWEEP>
 __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
     float* H, float* T, const float* __restrict__ HT0, const float* __restrict__ T0,
     const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ first_row,
     uint64_t n_windows, const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
-    double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A, unsigned long long* __restrict__ stats) {
+    double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A, const SweepFit* __restrict__ fits,
+    const SweepEpoch* __restrict__ fit_epoch, uint32_t nfits, unsigned long long* __restrict__ stats) {
   constexpr int NW = BT / 64;         // waves per block
   constexpr uint32_t PASS = 4 * BT;   // edges scanned per pass (4 a thread)
   constexpr uint32_t RING = 8 * BT;   // queue of active edges (holds < BT + PASS)
@@ -852,10 +827,14 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
   __shared__ float s_dh[BT][D], s_hk[BT][D];  // per lane: head change, twin tail change
   __shared__ uint32_t s_cnt[NW];
   __shared__ RowCarry<D> s_carry[2];
+  __shared__ uint32_t s_roff[SWEEP ? kMaxSweepFits + 1 : 1];  // sweep: first row of each fit
   if (threadIdx.x < 32) {
     const LibmTables c = libm_default_tables();
     s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
     s_log2[threadIdx.x] = c.log2[threadIdx.x];
+  }
+  if constexpr (SWEEP) {
+    for (uint32_t f = threadIdx.x; f <= nfits; f += BT) s_roff[f] = f < nfits ? fits[f].row_off : 0xffffffffu;
   }
   const LibmTables tb{s_exp2, s_log2};
   const LaneFit F{A.a, A.b, A.m2ab, A.g2b, A.alpha, A.n, A.ref_den, 0u, A.epoch_key};
@@ -957,6 +936,20 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
           const uint32_t e = s_e[slot];
           j = s_j[slot];
           const uint32_t k = __builtin_nontemporal_load(tail + e);
+          // this edge's fit: the whole problem, or the sweep member owning row j
+          LaneFit FL = F;
+          uint64_t e_local = e;
+          if constexpr (SWEEP) {
+            uint32_t lo = 0, hi = nfits;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (s_roff[mid] <= j) lo = mid; else hi = mid;
+            }
+            const SweepFit fp = fits[lo];
+            const SweepEpoch fe = fit_epoch[lo];
+            FL = LaneFit{fp.a, fp.b, fp.m2ab, fp.g2b, fe.alpha, fp.n, fp.ref_den, fp.row_off, fe.epoch_key};
+            e_local = e - fp.edge_off;
+          }
           double2 sc, rt;
           sc.x = __builtin_nontemporal_load(&sched[e].x);
           sc.y = __builtin_nontemporal_load(&sched[e].y);
@@ -968,12 +961,12 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
           __builtin_nontemporal_store(sc.x, &sched[e].x);
           __builtin_nontemporal_store(sc.y, &sched[e].y);
           __builtin_nontemporal_store(next_epoch<NextT>(sc.x), next + e);
-          const uint64_t ekey = rng_edge_key(F.epoch_key, e);
+          const uint64_t ekey = rng_edge_key(FL.epoch_key, e_local);
           uint32_t self_mask = 0;
           float tn[kPrefetch][D];
 #pragma unroll
           for (int q = 0; q < kPrefetch; ++q) {
-            const uint32_t kq = rng_index(ekey, (uint64_t)q, F.n);
+            const uint32_t kq = FL.row_off + rng_index(ekey, (uint64_t)q, FL.n);
             self_mask |= (kq == j) ? (1u << q) : 0u;
             if ((uint32_t)q < nn) {
 #pragma unroll
@@ -1004,11 +997,11 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
               diff[d] = h0[d] - tk[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+            const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
             float ga[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-              ga[d] = clip4(gc * diff[d]) * F.alpha;
+              ga[d] = clip4(gc * diff[d]) * FL.alpha;
               h[d] = h0[d] + ga[d];
             }
             if (!TWIN) tail_sub<D>(T + (uint64_t)k * D, tk, ga);  // this edge's tail side, now
@@ -1025,7 +1018,7 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
                   for (int d = 0; d < D; ++d) o[d] = tn[q][d];
                 }
             } else {
-              const uint32_t kp = rng_index(ekey, p, F.n);
+              const uint32_t kp = FL.row_off + rng_index(ekey, p, FL.n);
               skip = (kp == j);
 #pragma unroll
               for (int d = 0; d < D; ++d) o[d] = TWIN ? T0[(uint64_t)kp * D + d] : T[(uint64_t)kp * D + d];
@@ -1037,10 +1030,10 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
               diff[d] = h[d] - o[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float g2 = (d2 > 0.0f) ? repel_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+            const float g2 = (d2 > 0.0f) ? repel_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
             if (g2 > 0.0f) {
 #pragma unroll
-              for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * F.alpha;
+              for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * FL.alpha;
             }
           }
           float dt[D];
@@ -1053,9 +1046,9 @@ __global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
               diff[d] = hk[d] - t0[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float gc = (d2 > 0.0f) ? attract_coeff(F, d2, libm_powf(d2, F.b, tb)) : 0.0f;
+            const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
 #pragma unroll
-            for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * F.alpha);
+            for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * FL.alpha);
           }
 #pragma unroll
           for (int d = 0; d < D; ++d) {
@@ -1192,8 +1185,8 @@ void launch_edges(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   unsigned long long* stt = st->stats.as<unsigned long long>();
 #define UMAP_EDGES_CASE(DD)                                                                                       \
   case DD:                                                                                                        \
-    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL, false><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot,   \
-                                                                         cap, A, nullptr, nullptr, 0u, stt);      \
+    sgd_edges_kernel<DD, NextT, ATOMIC_TAIL><<<grid, 256, 0, s>>>(H, T, hd, tl, rt, sc, nx, li, ct, slot, cap, A, \
+                                                                  stt);                                         \
     break;
   switch (st->dim) {
     UMAP_EDGES_CASE(1)
@@ -1340,6 +1333,30 @@ uint64_t owner_window(uint64_t n_edges) {
   return w;
 }
 
+// Does every edge have a twin with a bit-identical schedule?
+bool twins_match(const uint64_t* row_ptr, const uint32_t* head, const uint32_t* tail, const double2* rate,
+                 const double2* sched, uint64_t E, hipStream_t s) {
+  if (E == 0) return false;
+  DevBuf bad(4);
+  UMAP_HIP(hipMemsetAsync(bad.p, 0, 4, s));
+  twin_check_kernel<<<grid_for(E, 256), 256, 0, s>>>(row_ptr, head, tail, rate, sched, E, bad.as<unsigned int>());
+  UMAP_LAUNCH_CHECK();
+  uint32_t hb = 1;
+  UMAP_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  return hb == 0;
+}
+
+// first_row of the windows of `size` edges over rows [0, n).
+DevBuf make_windows(const uint64_t* row_ptr, uint64_t n, uint64_t E, uint64_t size, uint64_t* count, hipStream_t s) {
+  *count = ceil_div(E, size);
+  if (*count > 0xffffffffull / kXcds) throw Error(Err::Unsupported, "too many edges for the SGD windows");
+  DevBuf fr((*count + 1) * 4);
+  list_blocks_kernel<<<grid_for(*count + 1, 256), 256, 0, s>>>(row_ptr, n, size, *count, fr.as<uint32_t>());
+  UMAP_LAUNCH_CHECK();
+  return fr;
+}
+
 // Row-owner Hogwild set-up: the twin check, then the windows (structure
 // only; fixed for the optimizer).
 void setup_owner(OptimizerState* st, hipStream_t s) {
@@ -1348,23 +1365,12 @@ void setup_owner(OptimizerState* st, hipStream_t s) {
   const uint64_t E = st->n_edges, n = st->n;
   if (E == 0 || n == 0 || n > 0xffffffffull) return;
   st->list_window = owner_window(E);
-  st->n_lblocks = ceil_div(E, st->list_window);
-  if (st->n_lblocks > 0xffffffffull / kXcds) throw Error(Err::Unsupported, "too many edges for the SGD windows");
-  st->lblock_row = DevBuf((st->n_lblocks + 1) * 4);
-  list_blocks_kernel<<<grid_for(st->n_lblocks + 1, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), n, st->list_window,
-                                                                       st->n_lblocks, st->lblock_row.as<uint32_t>());
-  UMAP_LAUNCH_CHECK();
-  {
-    DevBuf bad(4);
-    UMAP_HIP(hipMemsetAsync(bad.p, 0, 4, s));
-    twin_check_kernel<<<grid_for(E, 256), 256, 0, s>>>(st->row_ptr.as<uint64_t>(), st->head.as<uint32_t>(),
-                                                         st->tail.as<uint32_t>(), st->rate.as<double2>(),
-                                                         st->sched.as<double2>(), E, bad.as<unsigned int>());
-    UMAP_LAUNCH_CHECK();
-    uint32_t hb = 1;
-    UMAP_HIP(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
-    UMAP_HIP(hipStreamSynchronize(s));
-    st->twin_ok = (hb == 0);
+  st->lblock_row = make_windows(st->row_ptr.as<uint64_t>(), n, E, st->list_window, &st->n_lblocks, s);
+  st->twin_ok = twins_match(st->row_ptr.as<uint64_t>(), st->head.as<uint32_t>(), st->tail.as<uint32_t>(),
+                            st->rate.as<double2>(), st->sched.as<double2>(), E, s);
+  if (st->twin_ok) {  // epoch-start copies (both layouts: the buffer semantics can change later)
+    st->snap_h = DevBuf(std::max<uint64_t>(2 * n * st->dim, 1) * 4);
+    st->snap_t = DevBuf(std::max<uint64_t>(n * st->dim, 1) * 4);
   }
 }
 
@@ -1394,46 +1400,53 @@ void launch_fused(K kernel, uint64_t n_windows, hipStream_t s, Args... args) {
   kernel<<<grid, kFusedBlock, 0, s>>>(args...);
 }
 
-template <typename NextT, bool TWIN>
-void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
-  float* H = st->head_emb.as<float>();
-  float* T = tail_buffer(st);
-  const bool shared = (st->semantics & kSemanticsSharedBuffer) != 0;
-  const uint64_t* rp = st->row_ptr.as<uint64_t>();
-  const uint32_t* fr = st->lblock_row.as<uint32_t>();
-  const uint32_t* tl = st->tail.as<uint32_t>();
-  const double2* rt = st->rate.as<double2>();
-  double2* sc = st->sched.as<double2>();
-  NextT* nx = st->next.as<NextT>();
-  unsigned long long* stt = st->stats.as<unsigned long long>();
-  const uint64_t nw = st->n_lblocks;
+// One row-owner epoch over a set of rows and edges (one optimizer, or the
+// concatenated members of a sweep): the epoch-start copies, then the kernel.
+struct OwnerEpoch {
+  uint64_t n, dim, n_windows;
+  bool shared;
+  float *H, *T, *snap_ht, *snap_t;
+  const uint64_t* row_ptr;
+  const uint32_t* first_row;
+  const uint32_t* tail;
+  const double2* rate;
+  double2* sched;
+  void* next;
+  const SweepFit* fits;
+  const SweepEpoch* fit_epoch;
+  uint32_t nfits;
+  unsigned long long* stats;
+};
+
+template <typename NextT, bool TWIN, bool SWEEP>
+void run_owner_epoch(const OwnerEpoch& o, const SgdArgs& A, hipStream_t s) {
   const float* HT0 = nullptr;
   const float* T0 = nullptr;
   if (TWIN) {  // epoch-start copies
-    const uint64_t nel = st->n * st->dim;
-    const unsigned cgrid = (unsigned)std::min<uint64_t>(grid_for(nel, 256), 4096);
-    if (st->snap_t.bytes < nel * 4) st->snap_t = DevBuf(nel * 4);
-    if (shared) {
-      copy_f32_kernel<<<cgrid, 256, 0, s>>>(st->snap_t.as<float>(), H, nel);
+    const uint64_t nel = o.n * o.dim;
+    const unsigned cgrid = (unsigned)std::min<uint64_t>(grid_for(std::max<uint64_t>(nel, 1), 256), 4096);
+    if (o.shared) {
+      copy_f32_kernel<<<cgrid, 256, 0, s>>>(o.snap_t, o.H, nel);
     } else {
-      if (st->snap_h.bytes < 2 * nel * 4) st->snap_h = DevBuf(2 * nel * 4);
-      snapshot_kernel<<<cgrid, 256, 0, s>>>(st->snap_h.as<float>(), st->snap_t.as<float>(), H, T, st->n,
-                                            (uint32_t)st->dim);
-      HT0 = st->snap_h.as<float>();
+      snapshot_kernel<<<cgrid, 256, 0, s>>>(o.snap_ht, o.snap_t, o.H, o.T, o.n, (uint32_t)o.dim);
+      HT0 = o.snap_ht;
     }
     UMAP_LAUNCH_CHECK();
-    T0 = st->snap_t.as<float>();
+    T0 = o.snap_t;
   }
-#define UMAP_FUSED_CASE(DD)                                                                  \
-  case DD:                                                                                   \
-    if (shared)                                                                              \
-      launch_fused(sgd_fused_kernel<DD, NextT, true, TWIN, kFusedBlock>, nw, s, H, T, HT0, T0, rp, fr, nw, tl, rt, sc, \
-                   nx, A, stt);                                                              \
-    else                                                                                     \
-      launch_fused(sgd_fused_kernel<DD, NextT, false, TWIN, kFusedBlock>, nw, s, H, T, HT0, T0, rp, fr, nw, tl, rt, sc, \
-                   nx, A, stt);                                                              \
+  NextT* nx = static_cast<NextT*>(o.next);
+#define UMAP_FUSED_CASE(DD)                                                                                       \
+  case DD:                                                                                                        \
+    if (o.shared)                                                                                                 \
+      launch_fused(sgd_fused_kernel<DD, NextT, true, TWIN, kFusedBlock, SWEEP>, o.n_windows, s, o.H, o.T, HT0, T0, \
+                   o.row_ptr, o.first_row, o.n_windows, o.tail, o.rate, o.sched, nx, A, o.fits, o.fit_epoch,      \
+                   o.nfits, o.stats);                                                                             \
+    else                                                                                                          \
+      launch_fused(sgd_fused_kernel<DD, NextT, false, TWIN, kFusedBlock, SWEEP>, o.n_windows, s, o.H, o.T, HT0,   \
+                   T0, o.row_ptr, o.first_row, o.n_windows, o.tail, o.rate, o.sched, nx, A, o.fits, o.fit_epoch,  \
+                   o.nfits, o.stats);                                                                             \
     break;
-  switch (st->dim) {
+  switch (o.dim) {
     UMAP_FUSED_CASE(1)
     UMAP_FUSED_CASE(2)
     UMAP_FUSED_CASE(3)
@@ -1447,6 +1460,27 @@ void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   }
 #undef UMAP_FUSED_CASE
   UMAP_LAUNCH_CHECK();
+}
+
+template <typename NextT, bool TWIN>
+void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  OwnerEpoch o{};
+  o.n = st->n;
+  o.dim = st->dim;
+  o.n_windows = st->n_lblocks;
+  o.shared = (st->semantics & kSemanticsSharedBuffer) != 0;
+  o.H = st->head_emb.as<float>();
+  o.T = tail_buffer(st);
+  o.snap_ht = st->snap_h.as<float>();
+  o.snap_t = st->snap_t.as<float>();
+  o.row_ptr = st->row_ptr.as<uint64_t>();
+  o.first_row = st->lblock_row.as<uint32_t>();
+  o.tail = st->tail.as<uint32_t>();
+  o.rate = st->rate.as<double2>();
+  o.sched = st->sched.as<double2>();
+  o.next = st->next.p;
+  o.stats = st->stats.as<unsigned long long>();
+  run_owner_epoch<NextT, TWIN, false>(o, A, s);
 }
 
 }  // namespace
@@ -1687,7 +1721,7 @@ namespace {
 unsigned grid_stride_blocks(uint64_t n) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), 8192)); }
 
 template <typename NextT>
-void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t bps, hipStream_t s) {
+void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t widx, hipStream_t s) {
   SgdArgs A{};
   A.n = (uint32_t)sw->total_n;
   A.epoch = epoch;
@@ -1704,68 +1738,62 @@ void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t 
     UMAP_LAUNCH_CHECK();
     return;
   }
-  const uint32_t slot = (uint32_t)(epoch & 1u);
-  const uint64_t cap = list_capacity(sw->total_e);
-  active_edges_kernel<NextT><<<grid_for(sw->total_e, 256), 256, 0, s>>>(
-      sw->next.as<NextT>(), sw->total_e, (NextT)epoch, sw->active.as<uint32_t>(), sw->counters.as<uint32_t>(), slot,
-      cap);
-  UMAP_LAUNCH_CHECK();
-  const unsigned grid = kShards * bps;
-#define UMAP_SWEEP_CASE(DD)                                                                                       \
-  case DD:                                                                                                        \
-    sgd_edges_kernel<DD, NextT, true, true><<<grid, 256, 0, s>>>(                                                 \
-        H, T, sw->head.as<uint32_t>(), sw->tail.as<uint32_t>(), sw->rate.as<double2>(), sw->sched.as<double2>(), \
-        sw->next.as<NextT>(), sw->active.as<uint32_t>(), sw->counters.as<uint32_t>(), slot, cap, A,            \
-        sw->fits.as<SweepFit>(), fe, nf, stt);                                                                    \
-    break;
-  switch (sw->dim) {
-    UMAP_SWEEP_CASE(1)
-    UMAP_SWEEP_CASE(2)
-    UMAP_SWEEP_CASE(3)
-    UMAP_SWEEP_CASE(4)
-    UMAP_SWEEP_CASE(5)
-    UMAP_SWEEP_CASE(6)
-    UMAP_SWEEP_CASE(7)
-    UMAP_SWEEP_CASE(8)
-    default:
-      throw Error(Err::Unsupported, "a Hogwild sweep supports n_components <= 8");
-  }
-#undef UMAP_SWEEP_CASE
-  UMAP_LAUNCH_CHECK();
+  const SweepState::Windows& wd = sw->windows[widx];
+  OwnerEpoch o{};
+  o.n = sw->total_n;
+  o.dim = sw->dim;
+  o.n_windows = wd.count;
+  o.shared = (sw->semantics & kSemanticsSharedBuffer) != 0;
+  o.H = H;
+  o.T = T;
+  o.snap_ht = sw->snap_h.as<float>();
+  o.snap_t = sw->snap_t.as<float>();
+  o.row_ptr = sw->row_ptr.as<uint64_t>();
+  o.first_row = wd.first_row.as<uint32_t>();
+  o.tail = sw->tail.as<uint32_t>();
+  o.rate = sw->rate.as<double2>();
+  o.sched = sw->sched.as<double2>();
+  o.next = sw->next.p;
+  o.fits = sw->fits.as<SweepFit>();
+  o.fit_epoch = fe;
+  o.nfits = nf;
+  o.stats = stt;
+  if (sw->twin_ok)
+    run_owner_epoch<NextT, true, true>(o, A, s);
+  else
+    run_owner_epoch<NextT, false, true>(o, A, s);
 }
 
-// Candidate SGD grids of the autotuner: 128 regions x blocks per region.
-// (sweep_read_stats is declared in optimize.hpp.)
-constexpr uint32_t kTuneBps[] = {4, 8, 16, 32};
+// Autotuner candidates: edges per row-owner window.
+constexpr uint64_t kTuneWindows[] = {2048, 4096, 8192, 16384};
 
 template <typename NextT>
 void sweep_run(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
   const uint32_t nf = (uint32_t)sw->members.size();
   const uint64_t e0 = sw->current_epoch;
-  // tail snapshot (optimizer.rs:314) and counters, as kernels
+  // tail snapshot (optimizer.rs:314), as a kernel
   if (!(sw->semantics & kSemanticsSharedBuffer) && sw->total_n)
     copy_f32_kernel<<<grid_stride_blocks(sw->total_n * sw->dim), 256, 0, s>>>(
         sw->tail_emb.as<float>(), sw->head_emb.as<float>(), sw->total_n * sw->dim);
-  fill_u32(sw->counters.as<uint32_t>(), sw->counters.bytes / 4, 0u, s);
   uint64_t done = 0;
   const SweepEpoch* fe = sw->fit_epochs.as<SweepEpoch>();
-  // autotune: the first epochs of the first Hogwild call run one candidate
-  // grid each (every candidate computes a valid epoch), scored by device time
-  // per positive update since epochs differ in how many edges are active; an
-  // epoch without updates (epoch 0) does not score its candidate, which then
-  // gets the next epoch
+  // autotune: the first epochs of the first parallel call run one candidate
+  // window size each (every candidate computes the same epoch, bit for bit),
+  // scored by device time per positive update since epochs differ in how
+  // many edges are active; an epoch without updates (epoch 0) does not score
+  // its candidate, which then gets the next epoch
   if (!sw->tuned && sw->mode != SgdMode::Serial) {
     hipEvent_t a, b;
     UMAP_HIP(hipEventCreate(&a));
     UMAP_HIP(hipEventCreate(&b));
     float best = 0.0f;
-    const size_t ncand = sizeof(kTuneBps) / sizeof(kTuneBps[0]);
+    const size_t ncand = sw->windows.size();
     size_t c = 0;
     while (c < ncand && done + 1 < n_epochs) {
       uint64_t p0 = 0, p1 = 0, q = 0;
       sweep_read_stats(*sw, &p0, &q, s);
       UMAP_HIP(hipEventRecord(a, s));
-      sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, kTuneBps[c], s);
+      sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, (uint32_t)c, s);
       UMAP_HIP(hipEventRecord(b, s));
       UMAP_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
@@ -1774,10 +1802,10 @@ void sweep_run(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
       ++done;
       if (p1 == p0) continue;
       const float per = ms / (float)(p1 - p0);
-      sw->tuning.push_back({kTuneBps[c], ms, p1 - p0});
+      sw->tuning.push_back({sw->windows[c].size, ms, p1 - p0});
       if (best == 0.0f || per < best) {
         best = per;
-        sw->blocks_per_shard = kTuneBps[c];
+        sw->window_idx = (uint32_t)c;
       }
       ++c;
     }
@@ -1787,14 +1815,14 @@ void sweep_run(SweepState* sw, uint64_t n_epochs, hipStream_t s) {
   }
   if (done == n_epochs) return;
   if (!sw->use_graph) {
-    for (; done < n_epochs; ++done) sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, sw->blocks_per_shard, s);
+    for (; done < n_epochs; ++done) sweep_epoch<NextT>(sw, e0 + done, fe + done * nf, sw->window_idx, s);
     return;
   }
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   UMAP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    for (uint64_t e = done; e < n_epochs; ++e) sweep_epoch<NextT>(sw, e0 + e, fe + e * nf, sw->blocks_per_shard, s);
+    for (uint64_t e = done; e < n_epochs; ++e) sweep_epoch<NextT>(sw, e0 + e, fe + e * nf, sw->window_idx, s);
   } catch (...) {
     (void)hipStreamEndCapture(s, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1850,9 +1878,7 @@ void sweep_build(SweepState* sw, const std::vector<OptimizerState*>& members, hi
   sw->tail = DevBuf(std::max<uint64_t>(te, 1) * 4);
   sw->rate = DevBuf(std::max<uint64_t>(te, 1) * 16);
   sw->sched = DevBuf(std::max<uint64_t>(te, 1) * 16);
-  sw->next = DevBuf(std::max<uint64_t>(te, 1) * nb);
-  sw->active = DevBuf(list_capacity(te) * kShards * 4);
-  sw->counters = DevBuf(2 * kShards * kCounterStride * 4);
+  sw->next = DevBuf((ceil_div(te, 8) * 8 + 8) * nb);  // padded: the row-owner scan reads 4 counters at a time
   sw->stats = DevBuf(16);
   UMAP_HIP(hipMemsetAsync(sw->stats.p, 0, 16, s));
   sw->fits_host.clear();
@@ -1890,6 +1916,25 @@ void sweep_build(SweepState* sw, const std::vector<OptimizerState*>& members, hi
   sw->fits = DevBuf(sw->fits_host.size() * sizeof(SweepFit));
   UMAP_HIP(hipMemcpyAsync(sw->fits.p, sw->fits_host.data(), sw->fits_host.size() * sizeof(SweepFit),
                           hipMemcpyHostToDevice, s));
+  // row-owner structure of the concatenation: twin check, window candidates,
+  // epoch-start copies
+  sw->twin_ok = twins_match(sw->row_ptr.as<uint64_t>(), sw->head.as<uint32_t>(), sw->tail.as<uint32_t>(),
+                            sw->rate.as<double2>(), sw->sched.as<double2>(), te, s);
+  sw->windows.clear();
+  sw->tuning.clear();
+  sw->tuned = false;
+  sw->window_idx = 0;
+  if (te) {
+    for (uint64_t size : kTuneWindows) {
+      SweepState::Windows wd;
+      wd.size = size;
+      wd.first_row = make_windows(sw->row_ptr.as<uint64_t>(), tn, te, size, &wd.count, s);
+      sw->windows.push_back(std::move(wd));
+    }
+    sw->window_idx = 2;  // 8192 edges until the autotuner has run
+  }
+  sw->snap_h = DevBuf(std::max<uint64_t>(2 * tn * D, 1) * 4);
+  sw->snap_t = DevBuf(std::max<uint64_t>(tn * D, 1) * 4);
   UMAP_HIP(hipStreamSynchronize(s));
 }
 

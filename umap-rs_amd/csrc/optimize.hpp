@@ -123,17 +123,26 @@ struct SweepState {
   bool next16 = true;
   SgdMode mode = SgdMode::Hogwild;
   bool use_graph = true;
-  uint32_t blocks_per_shard = 16;  // SGD grid = 128 regions x blocks_per_shard (autotuned)
+  bool twin_ok = false;  // every member's schedules are twin-symmetric: row-owner synchronous epoch
+  // Row-owner windows: the autotuner's candidates for edges per window (one
+  // block walks one window at a time) and the one it picked.
+  struct Windows {
+    uint64_t size = 0, count = 0;
+    DevBuf first_row;  // u32[count+1]
+  };
+  std::vector<Windows> windows;
+  uint32_t window_idx = 0;
   bool tuned = false;
   struct Trial {
-    uint32_t blocks_per_shard;
-    float ms;           // device time of the epoch (list + SGD kernels)
-    uint64_t updates;   // positive updates in that epoch
+    uint64_t window;   // edges per window
+    float ms;          // device time of the epoch
+    uint64_t updates;  // positive updates in that epoch
   };
   std::vector<Trial> tuning;
   std::vector<SweepFit> fits_host;
   DevBuf fits, fit_epochs;
-  DevBuf head_emb, tail_emb, row_ptr, head, tail, rate, sched, next, active, counters, stats;
+  DevBuf head_emb, tail_emb, row_ptr, head, tail, rate, sched, next, stats;
+  DevBuf snap_h, snap_t;  // epoch-start copies (rows {H, T} side by side; T alone)
   double last_step_ms = 0.0;
   uint64_t graph_nodes = 0;
 };

@@ -846,7 +846,7 @@ class Sweep(_Handle):
     def stats(self):
         p, q, ms, b = u64(), u64(), f64(), u32()
         _check(lib().umapgpu_sweep_stats(self._handle(), C.byref(p), C.byref(q), C.byref(ms), C.byref(b)))
-        return {"positive": p.value, "negative": q.value, "last_step_ms": ms.value, "blocks_per_region": b.value}
+        return {"positive": p.value, "negative": q.value, "last_step_ms": ms.value, "edges_per_window": b.value}
 
     def tuning(self):
         import json
