@@ -730,7 +730,10 @@ __global__ void twin_check_kernel(const uint64_t* __restrict__ row_ptr, const ui
 // written once, by its owner: no list, no counters, no atomics on the
 // embedding.  Windows are dealt to XCDs in eighths (block b runs on XCD
 // b % 8), so each XCD's L2 keeps its own rows from one epoch to the next.
-constexpr uint32_t kFusedRowsLds = 1024;  // row pointers staged in LDS (else read from memory)
+#ifndef UMAP_ROWS_LDS
+#define UMAP_ROWS_LDS 1024
+#endif
+constexpr uint32_t kFusedRowsLds = UMAP_ROWS_LDS;  // row pointers staged in LDS (else read from memory)
 #ifndef UMAP_SGD_BT
 #define UMAP_SGD_BT 256
 #endif
