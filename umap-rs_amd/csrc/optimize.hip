@@ -8,20 +8,25 @@
 //   step_epochs                optimizer.rs:297-366
 //   Euclidean SGD epoch        layout/optimize_layout_euclidean.rs:236-347
 //
-// SGD design.  The reference's Rayon loop hands each thread a contiguous
-// range of edges; since edges are in CSR row order, one thread walks a head
-// row's edges in order and H[j] only ever changes under that thread.  Here
-// one lane owns one head row for the whole epoch: H[j] lives in registers
-// (read once, written once), the lane walks the row's edges in order, and
-// the positive and negative samples run through one flattened loop so that
-// the 64 lanes of a wave stay busy whatever their rows' activity pattern.
-// Rows are dealt to lanes by descending degree so a wave's lanes finish
-// together.  The tail snapshot is shared by all lanes: its attractive
-// updates are f32 hardware atomics (or plain racy stores, as the reference).
-// The f64 schedule recurrence is evaluated exactly as the reference does, so
-// every positive update and every negative-sample count is identical, and
-// powf is the glibc-equivalent libm_powf, so the serial mode reproduces the
-// sequential CPU run bit for bit.
+// SGD modes (all keep the reference's f64 schedule recurrence exactly, so
+// every positive update and negative-sample count equals the reference's):
+//   default      sgd_fused_kernel, one launch per epoch.  Blocks own windows
+//                of whole rows, compact each window's active edges in LDS and
+//                run one lane per edge.  On twin-symmetric schedules every read
+//                comes from a copy taken at the start of the epoch and each lane
+//                also evaluates its twin edge's tail side, so both ends of an
+//                edge move by the same gradient and each row is written once by
+//                its owner: no atomics, bit-reproducible (sgd_fused_kernel's
+//                comment has the details).  Otherwise tail sides go to the
+//                live snapshot with compare-and-swap.
+//   edge-atomic  active_edges_kernel lists the active edges, sgd_edges_kernel
+//                runs one lane per edge with atomic head and tail updates.
+//   rows         sgd_rows_kernel: one lane per head row walks its edges in
+//                order with the row in registers (the reference's per-thread
+//                order).
+//   serial       sgd_serial_kernel: the reference's edge order on one lane,
+//                bit-identical to the sequential CPU run (powf is the
+//                glibc-equivalent libm_powf).
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -403,44 +408,14 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64
 // XCD's L2 then only holds its eighth of the head embedding.
 constexpr uint32_t kShards = 128;
 constexpr uint32_t kCounterStride = 32;  // u32 words between counters (128 B)
-#ifndef UMAP_SGD_BPS
-#define UMAP_SGD_BPS 16
-#endif
-constexpr uint32_t kBlocksPerShard = UMAP_SGD_BPS;  // SGD blocks walking one region
-#ifndef UMAP_SGD_PREFETCH
-#define UMAP_SGD_PREFETCH 6
-#endif
-#ifndef UMAP_SGD_LDS_TABLES
-#define UMAP_SGD_LDS_TABLES 1
-#endif
-#ifndef UMAP_SGD_MIN_BLOCKS
-#define UMAP_SGD_MIN_BLOCKS 1
-#endif
-#ifndef UMAP_SGD_GATHER
-#define UMAP_SGD_GATHER 0
-#endif
-#ifndef UMAP_SGD_TAIL_CAS
-#define UMAP_SGD_TAIL_CAS 1
-#endif
-constexpr int kPrefetch = UMAP_SGD_PREFETCH;  // negative samples gathered up front
+constexpr uint32_t kBlocksPerShard = 16;  // SGD blocks walking one region
+constexpr int kPrefetch = 6;  // negative samples gathered up front (the mean count is the negative rate, 5)
 
-// One row of the tail snapshot.  UMAP_SGD_GATHER selects the load's cache
-// policy: 0 default, 1 non-temporal, 2..4 buffer loads with sc0|sc1, sc1,
-// sc0 (the buffer's range check returns 0 past the end; rows are < n).
+// One row of the tail snapshot.
 template <int D>
-__device__ inline void load_trow(const float* __restrict__ T, __amdgpu_buffer_rsrc_t rsrc, uint32_t row, float* out) {
-#if UMAP_SGD_GATHER == 0
+__device__ inline void load_trow(const float* __restrict__ T, uint32_t row, float* out) {
 #pragma unroll
   for (int d = 0; d < D; ++d) out[d] = T[(uint64_t)row * D + d];
-#elif UMAP_SGD_GATHER == 1
-#pragma unroll
-  for (int d = 0; d < D; ++d) out[d] = __builtin_nontemporal_load(T + (uint64_t)row * D + d);
-#else
-  constexpr int aux = UMAP_SGD_GATHER == 2 ? 17 : (UMAP_SGD_GATHER == 3 ? 16 : 1);
-#pragma unroll
-  for (int d = 0; d < D; ++d)
-    out[d] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)((row * D + d) * 4u), 0, aux));
-#endif
 }
 
 // Step 1: list the edges active this epoch (next[e] <= epoch).  Block 0 also
@@ -485,7 +460,7 @@ __global__ __launch_bounds__(256) void active_edges_kernel(const NextT* __restri
 // counter RNG -- is issued before any of them is used, so the lane waits
 // for one memory round trip instead of one per sample.
 template <int D, typename NextT, bool ATOMIC_TAIL>
-__global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
+__global__ __launch_bounds__(256) void sgd_edges_kernel(float* __restrict__ H, float* __restrict__ T,
                                                         const uint32_t* __restrict__ head,
                                                         const uint32_t* __restrict__ tail,
                                                         const double2* __restrict__ rate, double2* __restrict__ sched,
@@ -496,18 +471,16 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
   // powf's two lookup tables, staged in LDS for low-latency divergent reads
   __shared__ uint64_t s_exp2[32];
   __shared__ double s_log2[32];
-  if (UMAP_SGD_LDS_TABLES && threadIdx.x < 32) {
+  if (threadIdx.x < 32) {
     const LibmTables c = libm_default_tables();
     s_exp2[threadIdx.x] = c.exp2[threadIdx.x];
     s_log2[threadIdx.x] = c.log2[threadIdx.x];
   }
   __syncthreads();
-  const LibmTables tb = UMAP_SGD_LDS_TABLES ? LibmTables{s_exp2, s_log2} : libm_default_tables();
+  const LibmTables tb{s_exp2, s_log2};
   const uint32_t shard = blockIdx.x % kShards, sub = blockIdx.x / kShards;
   const uint64_t count = counters[(slot * kShards + shard) * kCounterStride];
   const uint32_t* lst = list + (uint64_t)shard * cap;
-  const __amdgpu_buffer_rsrc_t trsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void*)T, (short)0, (int)(A.n * (uint32_t)D * 4u), 0x00020000);
   const uint64_t stride = (uint64_t)(gridDim.x / kShards) * blockDim.x;
   const int lane = threadIdx.x & 63;
   uint64_t npos = 0, nneg = 0;
@@ -545,12 +518,12 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
       for (int q = 0; q < kPrefetch; ++q) {
         const uint32_t kq = F.row_off + rng_index(ekey, (uint64_t)q, F.n);
         self_mask |= (kq == j) ? (1u << q) : 0u;
-        if ((uint32_t)q < nn) load_trow<D>(T, trsrc, kq, tn[q]);
+        if ((uint32_t)q < nn) load_trow<D>(T, kq, tn[q]);
       }
       float h[D], tv[D], diff[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) h[d] = H[(uint64_t)j * D + d];
-      load_trow<D>(T, trsrc, k, tv);
+      load_trow<D>(T, k, tv);
 #pragma unroll
       for (int d = 0; d < D; ++d) delta[d] = h[d];
       {  // attractive sample (:278-306)
@@ -567,7 +540,7 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
           ga[d] = clip4(gc * diff[d]) * F.alpha;
           h[d] = h[d] + ga[d];
         }
-        if (ATOMIC_TAIL && UMAP_SGD_TAIL_CAS && D == 2) {
+        if (ATOMIC_TAIL && D == 2) {
           // both coordinates in one 64-bit compare-and-swap
           unsigned long long* tp = reinterpret_cast<unsigned long long*>(T + (uint64_t)k * D);
           unsigned long long expect = ((unsigned long long)__float_as_uint(tv[D > 1 ? 1 : 0]) << 32) |
@@ -606,7 +579,7 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
         } else {
           const uint32_t kp = F.row_off + rng_index(ekey, p, F.n);
           skip = (kp == j);
-          load_trow<D>(T, trsrc, kp, o);
+          load_trow<D>(T, kp, o);
         }
         if (skip) continue;
         float d2 = 0.0f;
@@ -653,8 +626,9 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
 }
 
 // ---------------------------------------------------------------------------
-// Row-owner Hogwild (the default when every edge's twin has a bit-identical
-// schedule, which holds for the symmetric graphs learn_manifold builds).
+// Row-owner synchronous epoch (the default when every edge's twin has a
+// bit-identical schedule, which holds for the symmetric graphs
+// learn_manifold builds).
 //
 // Device-scope atomics on this memory are carried out at the memory side, so
 // the per-edge tail update of sgd_edges_kernel (a 64-bit compare-and-swap on
@@ -663,17 +637,19 @@ __global__ __launch_bounds__(256, UMAP_SGD_MIN_BLOCKS) void sgd_edges_kernel(flo
 // same epoch, and the twin's tail-side update lands on T[j]:
 //     T[j] -= clip(gc(|H[k] - T[j]|^2) * (H[k] - T[j])) * alpha
 // (optimize_layout_euclidean.rs:284-306 with the roles of j and k swapped).
-// So the lane that runs edge (j,k) also evaluates that twin update, reading
-// H[k] next to T[k].  Every write of an epoch then lands on the lane's own
-// head row j, for both arrays: a row's active edges are listed contiguously
-// and run inside one wave, whose run-tail lane stores H[j] and T[j] with
-// plain stores.  Only a row whose run crosses a wave boundary falls back to
-// f32 atomics.  Each active edge still makes exactly one head update and one
-// tail update per active epoch, with the counts and schedule of the
-// reference; what changes is which Hogwild interleaving occurs.
+// The lane that runs edge (j,k) evaluates that update too, so every write of
+// an epoch lands on the lane's own row j, which one block owns.  What the
+// reference's Hogwild loop always preserves is that one attractive step moves
+// both ends of an edge by the same gradient (:299-304); here every read comes
+// from copies of H and T taken at the start of the epoch, so the owner of k
+// (head side of (k,j)) and the owner of j (its tail side) compute that
+// gradient from the same two values, bit for bit.  Each active edge makes
+// exactly one head update and one tail update per active epoch, with the
+// reference's counts and schedule; no atomics, and the result does not
+// depend on scheduling.
 
-// Step 0 (Optimizer::new): list blocks hold whole rows, those whose first
-// edge falls in [b*W, (b+1)*W).
+// Step 0 (Optimizer::new): window b holds the whole rows whose first edge
+// falls in [b*W, (b+1)*W).
 __global__ void list_blocks_kernel(const uint64_t* __restrict__ row_ptr, uint64_t n, uint64_t window,
                                    uint64_t n_blocks, uint32_t* __restrict__ first_row) {
   const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -730,14 +706,10 @@ __global__ void twin_check_kernel(const uint64_t* __restrict__ row_ptr, const ui
 // written once, by its owner: no list, no counters, no atomics on the
 // embedding.  Windows are dealt to XCDs in eighths (block b runs on XCD
 // b % 8), so each XCD's L2 keeps its own rows from one epoch to the next.
-#ifndef UMAP_ROWS_LDS
-#define UMAP_ROWS_LDS 1024
-#endif
-constexpr uint32_t kFusedRowsLds = UMAP_ROWS_LDS;  // row pointers staged in LDS (else read from memory)
-#ifndef UMAP_SGD_BT
-#define UMAP_SGD_BT 256
-#endif
-constexpr int kFusedBlock = UMAP_SGD_BT;  // threads per row-owner block
+constexpr uint32_t kFusedRowsLds = 1024;  // row pointers staged in LDS (else read from memory)
+// Threads per row-owner block; measured at 1M points: 256 threads 810 us per
+// epoch, 128 852, 64 877-1122 (with the window size tuned for each).
+constexpr int kFusedBlock = 256;
 
 // Reads of another row and the owner's writes go through the device's
 // coherence point (agent-scope relaxed atomics: sc1 loads and stores), so a
@@ -813,7 +785,7 @@ __device__ inline void finish_row(float* H, float* T, uint32_t j, const float* h
 }
 
 template <int D, typename NextT, bool SHARED, bool TWIN, int BT, bool SWEEP>
-__global__ __launch_bounds__(BT, UMAP_SGD_MIN_BLOCKS) void sgd_fused_kernel(
+__global__ __launch_bounds__(BT) void sgd_fused_kernel(
     float* H, float* T, const float* __restrict__ HT0, const float* __restrict__ T0,
     const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ first_row,
     uint64_t n_windows, const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
@@ -1327,12 +1299,9 @@ void alloc_edge_parallel(OptimizerState* st, hipStream_t s) {
 // over more chunks; measured at 1M points: 2048 edges 1339 us/epoch, 4096
 // 938, 8192 865.  Small graphs use smaller windows so that there are still
 // about 4096 blocks.
-#ifndef UMAP_WIN_MAX
-#define UMAP_WIN_MAX 8192
-#endif
 uint64_t owner_window(uint64_t n_edges) {
   uint64_t w = 256;
-  while (w < UMAP_WIN_MAX && w * 4096 < n_edges) w <<= 1;
+  while (w < 8192 && w * 4096 < n_edges) w <<= 1;
   return w;
 }
 
