@@ -282,11 +282,17 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
 /* Exact brute-force Euclidean kNN (the caller-side producer the reference
  * leaves to the user, README.md:37-43): row i of the output holds i itself
  * at slot 0 (distance 0), then its k-1 nearest other points by ascending
- * distance (ties by index).  data is n x dim row-major, all device
- * pointers. */
+ * distance (ties by index), distance = sqrt of sum((x_i - x_j)^2) summed in
+ * f32 in dimension order.  data is n x dim row-major, all device pointers.
+ * For dim >= 8 candidates are ranked on the matrix cores and re-ranked with
+ * that sum; an error bound proves each row exact or sends it to the scalar
+ * kernel, so the output does not depend on the path. */
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);
+/* Rows of the last umapgpu_knn_bruteforce_device call that the exactness
+ * bound sent to the scalar kernel (0 for dim < 8, which only uses it). */
+UMAPGPU_API uint64_t umapgpu_knn_last_fallback_rows(void);
 /* ---- batched sweep (no reference counterpart: the reference steps one
  * Optimizer at a time, optimizer.rs:297).  Many independent Euclidean fits
  * (hyperparameter sweeps, per-cluster sub-embeddings) step together: their

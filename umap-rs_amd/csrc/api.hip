@@ -892,9 +892,12 @@ umapgpu_status umapgpu_knn_bruteforce_device(const float* data, uint64_t n, uint
                                              float* dists) {
   return guarded([&] {
     ensure_device();
+    g_knn_fallback_queries = 0;
     knn_bruteforce(data, n, dim, k, idx, dists, stream());
   });
 }
+
+uint64_t umapgpu_knn_last_fallback_rows(void) { return g_knn_fallback_queries; }
 
 umapgpu_status umapgpu_sweep_create(umapgpu_optimizer* const* members, uint64_t count, umapgpu_sweep** out) {
   return guarded([&] {

@@ -16,7 +16,11 @@
 #include "common.hpp"
 #include "knn.hpp"
 
+#include <string.h>
+
 namespace umapgpu {
+
+extern uint64_t g_knn_fallback_queries;
 
 namespace {
 
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
     }
   }
   if (t >= nq) return;
-  const uint64_t o = (uint64_t)t * k;
+  const uint64_t o = (uint64_t)(SELF ? q : t) * k;  // SELF: row q of the n x k output
   if (SELF) {
     out_i[o] = q;
     if (out_d) out_d[o] = 0.0f;
@@ -97,6 +101,178 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
       out_i[o + (SELF ? 1 : 0) + i] = bi[i];
       if (out_d) out_d[o + (SELF ? 1 : 0) + i] = sqrtf(bd[i]);
     }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA producer (dim >= 8).  Squared distances are first ranked through
+// |q|^2 + |c|^2 - 2 q.c with the dot products on the matrix cores
+// (v_mfma_f32_16x16x4_f32: f32 inputs, f32 accumulation).  Each query keeps
+// its KM smallest approximate values (its k-1 neighbours plus a margin);
+// those candidates are then re-ranked with the same sum((q-c)^2), in the same
+// order, as knn_kernel, so indices and distances equal the scalar producer's.
+// The result is exact when the approximation cannot have dropped a closer
+// point: with an error bound delta on every approximate value, every point
+// outside the list has an exact distance >= (largest listed approximation -
+// delta), and that must exceed the (k-1)-th re-ranked distance.  Queries
+// that fail the check are listed and rerun by knn_kernel.
+//
+// Layout: a block is 4 waves x 32 queries; candidates are staged 32 at a
+// time in LDS transposed (dimension-major, row stride 48 floats so that the
+// four k-rows a B operand read touches fall on distinct banks).  MFMA lane
+// layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// receives C[4*(l/16) + r][l%16] in register r.
+constexpr int kMQ = 128;     // queries per block (32 per wave)
+constexpr int kMC = 32;      // candidates per tile
+constexpr int kMCS = 48;     // LDS row stride of the transposed candidate tile
+
+typedef float knn_v4f __attribute__((ext_vector_type(4)));
+
+template <int DIM, int KM>
+__global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__ X, const float* __restrict__ norms,
+                                                       uint32_t n, uint32_t k,
+                                                       float delta_scale, float cmax2, uint32_t* __restrict__ out_i,
+                                                       float* __restrict__ out_d, uint32_t* __restrict__ redo,
+                                                       uint32_t* __restrict__ redo_count) {
+  constexpr int DP = (DIM + 3) / 4 * 4;  // K padded to the MFMA step
+  constexpr int KS = DP / 4;             // MFMA steps per 16x16 tile
+  __shared__ float s_ct[DP * kMCS];      // candidates, transposed
+  __shared__ float s_cn[kMC];            // their squared norms
+  __shared__ float s_qn[kMQ];            // the block's query norms
+  __shared__ float s_d[4][32][kMC + 1];  // per wave: 32 queries x 32 approximate distances
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t qbase = blockIdx.x * kMQ + w * 32;
+  // A operands: rows qbase + t*16 + lane%16, dimension 4*s + lane/16
+  float a[2][KS];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const uint32_t qr = qbase + t * 16 + (lane & 15);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int d = 4 * s + (lane >> 4);
+      a[t][s] = (qr < n && d < DIM) ? X[(uint64_t)qr * DIM + d] : 0.0f;
+    }
+  }
+  if (threadIdx.x < kMQ) {
+    const uint32_t qr = blockIdx.x * kMQ + threadIdx.x;
+    s_qn[threadIdx.x] = qr < n ? norms[qr] : 0.0f;
+  }
+  // per query (lanes 0..31 of each wave): the KM smallest approximations
+  const uint32_t myq = qbase + lane;
+  const bool owner = lane < 32 && myq < n;
+  float bd[KM];
+  uint32_t bi[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    bd[i] = INFINITY;
+    bi[i] = 0xffffffffu;
+  }
+  float worst = INFINITY;
+  for (uint32_t c0 = 0; c0 < n; c0 += kMC) {
+    __syncthreads();  // the previous tile is done with s_ct (and s_qn is written)
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(kMC * DP); e += 256) {
+      const uint32_t c = e / DP, d = e - c * DP;
+      const uint32_t ci = c0 + c;
+      s_ct[d * kMCS + c] = (ci < n && d < (uint32_t)DIM) ? X[(uint64_t)ci * DIM + d] : 0.0f;
+    }
+    if (threadIdx.x < kMC) s_cn[threadIdx.x] = (c0 + threadIdx.x < n) ? norms[c0 + threadIdx.x] : 0.0f;
+    __syncthreads();
+    knn_v4f acc[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[t][u] = knn_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int kr = 4 * s + (lane >> 4);
+      const float b0 = s_ct[kr * kMCS + (lane & 15)];
+      const float b1 = s_ct[kr * kMCS + 16 + (lane & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][s], b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][s], b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], b1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = t * 16 + 4 * (lane >> 4) + r, col = u * 16 + (lane & 15);
+          s_d[w][row][col] = (s_qn[w * 32 + row] + s_cn[col]) - 2.0f * acc[t][u][r];
+        }
+    __syncthreads();  // s_d is ready
+    if (owner) {
+      const uint32_t nc = (n - c0) < (uint32_t)kMC ? (n - c0) : (uint32_t)kMC;
+#pragma unroll 8
+      for (uint32_t c = 0; c < nc; ++c) {
+        const float v = s_d[w][lane][c];
+        const uint32_t ci = c0 + c;
+        if (v < worst && ci != myq) {
+          float nd = v;
+          uint32_t ni = ci;
+#pragma unroll
+          for (int i = 0; i < KM; ++i) {
+            const bool sw = nd < bd[i] || (nd == bd[i] && ni < bi[i]);
+            const float td = bd[i];
+            const uint32_t ti = bi[i];
+            bd[i] = sw ? nd : td;
+            bi[i] = sw ? ni : ti;
+            nd = sw ? td : nd;
+            ni = sw ? ti : ni;
+          }
+          worst = bd[KM - 1];
+        }
+      }
+    }
+  }
+  if (!owner) return;
+  // re-rank the listed candidates exactly (knn_kernel's sum and order)
+  const float approx_max = bd[KM - 1];
+  const bool listed_all = bi[KM - 1] == 0xffffffffu;  // fewer than KM other points exist
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    float acc = INFINITY;
+    if (bi[i] != 0xffffffffu) {
+      acc = 0.0f;
+      for (int d = 0; d < DIM; ++d) {
+        const float df = X[(uint64_t)myq * DIM + d] - X[(uint64_t)bi[i] * DIM + d];
+        acc = acc + df * df;
+      }
+    }
+    bd[i] = acc;
+  }
+  // sort by (exact, index): insertion sort with compile-time positions
+#pragma unroll
+  for (int i = 1; i < KM; ++i)
+#pragma unroll
+    for (int j = i; j > 0; --j) {
+      const bool sw = bd[j] < bd[j - 1] || (bd[j] == bd[j - 1] && bi[j] < bi[j - 1]);
+      const float td = bd[j - 1];
+      const uint32_t ti = bi[j - 1];
+      bd[j - 1] = sw ? bd[j] : td;
+      bi[j - 1] = sw ? bi[j] : ti;
+      bd[j] = sw ? td : bd[j];
+      bi[j] = sw ? ti : bi[j];
+    }
+  // exactness check: every unlisted point lies at >= approx_max - delta
+  const uint32_t kk = k - 1;
+  const float qn = s_qn[w * 32 + lane];
+  const float delta = delta_scale * (qn + cmax2 + 2.0f * sqrtf(qn * cmax2));
+  float kth = 0.0f;
+#pragma unroll
+  for (int q = 0; q < KM; ++q)
+    if ((uint32_t)q + 1 == kk) kth = bd[q];
+  const bool proven = kk == 0 || listed_all || (kth < approx_max - delta);
+  const uint64_t o = (uint64_t)myq * k;
+  out_i[o] = myq;
+  if (out_d) out_d[o] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < KM; ++q)
+    if ((uint32_t)q < kk) {
+      out_i[o + 1 + q] = bi[q];
+      if (out_d) out_d[o + 1 + q] = sqrtf(bd[q]);
+    }
+  if (!proven) redo[atomicAdd(redo_count, 1u)] = myq;
 }
 
 template <int DIM, bool SELF>
@@ -113,7 +289,62 @@ void launch_dim(const float* X, uint64_t n, uint64_t k, const uint32_t* queries,
   UMAP_LAUNCH_CHECK();
 }
 
+// Squared row norms (sum of squares in dimension order) and their maximum,
+// as float bits (non-negative floats order like their bit patterns).
+template <int DIM>
+__global__ void norms_kernel(const float* __restrict__ X, uint32_t n, float* __restrict__ norms,
+                             unsigned int* __restrict__ out) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  float acc = 0.0f;
+  if (r < n) {
+    for (int d = 0; d < DIM; ++d) {
+      const float v = X[(uint64_t)r * DIM + d];
+      acc = acc + v * v;
+    }
+    norms[r] = acc;
+  }
+  unsigned int b = __float_as_uint(acc);
+  for (int off = 32; off > 0; off >>= 1) b = max(b, (unsigned int)__shfl_xor((int)b, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, b);
+}
+
+// MFMA producer with the scalar kernel as the fallback for the queries whose
+// exactness the error bound cannot prove.
+template <int DIM>
+void launch_mfma(const float* X, uint64_t n, uint64_t k, uint32_t* oi, float* od, hipStream_t s) {
+  DevBuf mx(4), cnt(4), redo(std::max<uint64_t>(n, 1) * 4), norms(std::max<uint64_t>(n, 1) * 4);
+  UMAP_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+  UMAP_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+  norms_kernel<DIM><<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(X, (uint32_t)n, norms.as<float>(),
+                                                                mx.as<unsigned int>());
+  UMAP_LAUNCH_CHECK();
+  unsigned int mb = 0;
+  UMAP_HIP(hipMemcpyAsync(&mb, mx.p, 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  float cmax2 = 0.0f;
+  memcpy(&cmax2, &mb, 4);
+  if (!(cmax2 < INFINITY)) throw Error(Err::InvalidArgument, "kNN input has non-finite values");
+  cmax2 = cmax2 * 1.0001f;
+  // |approx - f32 sum((q-c)^2)| <= 2 gamma (|q|^2 + |c|^2 + 2|q||c|), gamma = (D + 4) 2^-24, doubled
+  const float delta_scale = 4.0f * (float)(DIM + 4) * 5.9604645e-8f;
+  const unsigned grid = (unsigned)ceil_div(n, kMQ);
+  if (k <= 16)
+    knn_mfma_kernel<DIM, 32><<<grid, 256, 0, s>>>(X, norms.as<float>(), (uint32_t)n, (uint32_t)k, delta_scale, cmax2, oi, od,
+                                                  redo.as<uint32_t>(), cnt.as<uint32_t>());
+  else
+    knn_mfma_kernel<DIM, 48><<<grid, 256, 0, s>>>(X, norms.as<float>(), (uint32_t)n, (uint32_t)k, delta_scale, cmax2, oi, od,
+                                                  redo.as<uint32_t>(), cnt.as<uint32_t>());
+  UMAP_LAUNCH_CHECK();
+  uint32_t nredo = 0;
+  UMAP_HIP(hipMemcpyAsync(&nredo, cnt.p, 4, hipMemcpyDeviceToHost, s));
+  UMAP_HIP(hipStreamSynchronize(s));
+  g_knn_fallback_queries = nredo;
+  if (nredo) launch_dim<DIM, true>(X, n, k, redo.as<uint32_t>(), nredo, oi, od, s);
+}
+
 }  // namespace
+
+uint64_t g_knn_fallback_queries = 0;
 
 void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32_t* oi, float* od, hipStream_t s) {
   if (k < 1 || k > n) throw Error(Err::InvalidArgument, "kNN needs 1 <= k <= n");
@@ -121,12 +352,12 @@ void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32
   switch (dim) {
     case 2: launch_dim<2, true>(X, n, k, nullptr, n, oi, od, s); break;
     case 3: launch_dim<3, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 8: launch_dim<8, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 16: launch_dim<16, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 20: launch_dim<20, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 32: launch_dim<32, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 64: launch_dim<64, true>(X, n, k, nullptr, n, oi, od, s); break;
-    case 128: launch_dim<128, true>(X, n, k, nullptr, n, oi, od, s); break;
+    case 8: launch_mfma<8>(X, n, k, oi, od, s); break;
+    case 16: launch_mfma<16>(X, n, k, oi, od, s); break;
+    case 20: launch_mfma<20>(X, n, k, oi, od, s); break;
+    case 32: launch_mfma<32>(X, n, k, oi, od, s); break;
+    case 64: launch_mfma<64>(X, n, k, oi, od, s); break;
+    case 128: launch_mfma<128>(X, n, k, oi, od, s); break;
     default: throw Error(Err::Unsupported, "brute-force kNN supports dim in {2,3,8,16,20,32,64,128}");
   }
   UMAP_HIP(hipStreamSynchronize(s));

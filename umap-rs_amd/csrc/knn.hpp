@@ -5,7 +5,11 @@
 
 namespace umapgpu {
 
-// X: n x dim row-major device array; outputs n x k (self at slot 0).
+// X: n x dim row-major device array; outputs n x k (self at slot 0).  For
+// dim >= 8 the distances are ranked on the matrix cores and re-ranked
+// exactly; g_knn_fallback_queries counts the queries of the last call that
+// the exactness bound sent back to the scalar kernel.
+extern uint64_t g_knn_fallback_queries;
 void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32_t* out_idx, float* out_dist,
                     hipStream_t s);
 

@@ -138,6 +138,7 @@ def lib():
             "umapgpu_memcpy_dtoh": ([P, P, u64], i32),
             "umapgpu_synchronize": ([], i32),
             "umapgpu_knn_bruteforce_device": ([P, u64, u64, u64, P, P], i32),
+            "umapgpu_knn_last_fallback_rows": ([], u64),
             "umapgpu_fuzzy_cross_entropy_device": ([P, P, P, u64, P, u64, f32, f32, u64, u64,
                                                     C.POINTER(_CeTerms)], i32),
             "umapgpu_manifold_fuzzy_cross_entropy": ([P, P, u64, u64, u64, C.POINTER(_CeTerms)], i32),
@@ -814,6 +815,12 @@ def knn_bruteforce_device(data: DeviceArray, k: int):
     dst = DeviceArray((n, k), np.float32)
     _check(lib().umapgpu_knn_bruteforce_device(P(data.ptr), n, dim, k, P(idx.ptr), P(dst.ptr)))
     return idx, dst
+
+
+def knn_last_fallback_rows() -> int:
+    """Rows of the last knn_bruteforce_device call that the exactness bound
+    sent to the scalar kernel."""
+    return int(lib().umapgpu_knn_last_fallback_rows())
 
 
 class Sweep(_Handle):
