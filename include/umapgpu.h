@@ -189,6 +189,19 @@ UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs(umapgpu_optimizer* opt,
  * caller's Metric::distance for every sample. */
 UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* opt, uint64_t n,
                                                                  umapgpu_distance_fn distance, void* user);
+/* Optimizer::step_epochs(n, &EuclideanMetric) (optimizer.rs:297-366) with
+ * the metric built into the device code instead of a callback.  For
+ * MetricType::Generic it runs the generic loop
+ * (layout/optimize_layout_generic.rs:35-124) with EuclideanMetric::distance
+ * (distances.rs:22-38) on the device: SERIAL is bit-identical to the
+ * reference's sequential loop; the parallel modes run the row-owner epoch
+ * with the generic forms (n_components <= 8, else UMAPGPU_UNSUPPORTED).
+ * For MetricType::Euclidean it is umapgpu_optimizer_step_epochs, as in the
+ * reference, where the metric argument is then unused.  metric: one of
+ * UMAPGPU_BUILTIN_METRIC_*. */
+#define UMAPGPU_BUILTIN_METRIC_EUCLIDEAN 0
+UMAPGPU_API umapgpu_status umapgpu_optimizer_step_epochs_builtin(umapgpu_optimizer* opt, uint64_t n,
+                                                                 int32_t metric);
 UMAPGPU_API umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* opt, int32_t mode);
 /* Opt-in SGD semantics (no reference counterpart; default 0 = the
  * reference).  Bits:

@@ -221,14 +221,16 @@ class OptimizerState:
 
     def step_epochs(self, n_epochs: int, mode: str = "serial"):
         """Optimizer::step_epochs (optimizer.rs:297-366); mode 'serial', 'hogwild', or 'sync' (the
-        build's row-owner synchronous epoch, restated for bit-level checks of the device default)."""
+        build's row-owner synchronous epoch, restated for bit-level checks of the device default);
+        'generic_serial' and 'generic_sync' are the same two epochs with MetricType::Generic and the
+        Euclidean metric (optimize_layout_generic.rs:35-124, distances.rs:22-38)."""
         if self.current_epoch + n_epochs > self.total_epochs:
             raise ValueError(f"Cannot step {n_epochs} epochs: would exceed total_epochs {self.total_epochs} "
                              f"(current: {self.current_epoch})")
         lib().orc_step_epochs(_ptr(self.embedding), self.n, self.dim, _ptr(self.head), _ptr(self.tail),
                               self.n_edges, _ptr(self.eps), _ptr(self.eons), _ptr(self.epns), _ptr(self.eonns),
                               self.a, self.b, self.gamma, self.initial_alpha, self.current_epoch, n_epochs,
-                              self.total_epochs, self.seed, {"serial": 0, "hogwild": 1, "sync": 2}[mode], _ptr(self.counts),
+                              self.total_epochs, self.seed, _STEP_MODES[mode], _ptr(self.counts),
                               self.semantics)
         self.current_epoch += n_epochs
 
@@ -243,6 +245,9 @@ class OptimizerState:
                                                _ptr(self.eons), _ptr(self.epns), _ptr(self.eonns), self.a, self.b,
                                                self.gamma, alpha, e, self.seed)
         self.current_epoch += n_epochs
+
+
+_STEP_MODES = {"serial": 0, "hogwild": 1, "sync": 2, "generic_sync": 3, "generic_serial": 4}
 
 
 def rng_index(seed: int, epoch: int, edge: int, p: int, n: int) -> int:

@@ -628,10 +628,27 @@ ORC_API void orc_sgd_epoch_hogwild(float *head_emb, float *tail_emb, uint64_t di
  * and the row is written once: head[j] = h0[j] + ((0 + dh_1) + dh_2 ...),
  * tail[j] = ((t0[j] + dt_1) + dt_2) ...; with one shared array,
  * head[j] = (h0[j] + sum dh) + (t - t0[j]).                                 */
-ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, uint64_t dim, const uint32_t *head,
-                                const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices, const double *eps,
-                                double *eons, const double *epns, double *eonns, float a, float b, float gamma,
-                                float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts) {
+static float euclid_dist_grad(const float *x, const float *y, uint64_t dim, float *grad);
+static void sgd_epoch_generic_serial(float *head_emb, float *tail_emb, uint64_t dim, const uint32_t *head,
+                                     const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices, const double *eps,
+                                     double *eons, const double *epns, double *eonns, float a, float b, float gamma,
+                                     float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts);
+
+/* The generic-metric forms (optimize_layout_generic.rs:64-118) with the
+ * Euclidean metric's distance and gradient (distances.rs:25-41). */
+static inline float generic_attract_coeff(float dist, float a, float b) {
+  float w_l = (dist > 0.0f) ? 1.0f / (1.0f + a * powf(dist, 2.0f * b)) : 1.0f;
+  return ((2.0f * b) * (w_l - 1.0f)) / (dist + 1e-6f);
+}
+static inline float generic_repel_coeff(float dist, float a, float b, float gamma) {
+  float wl = (dist > 0.0f) ? 1.0f / (1.0f + a * powf(dist, 2.0f * b)) : 1.0f;
+  return (((gamma * 2.0f) * b) * wl) / (dist + 1e-6f);
+}
+
+static void sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, int generic, uint64_t dim,
+                           const uint32_t *head, const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices,
+                           const double *eps, double *eons, const double *epns, double *eonns, float a, float b,
+                           float gamma, float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts) {
   if (dim > 16) {
     fprintf(stderr, "orc_sgd_epoch_sync: dim %llu > 16\n", (unsigned long long)dim);
     abort();
@@ -662,48 +679,69 @@ ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, ui
       eons[i] = eons[i] + eps[i];
       const uint64_t n_neg = sat_u64(((double)epoch - eonns[i]) / epns[i]);
       eonns[i] = eonns[i] + (double)n_neg * epns[i];
-      float d2 = 0.0f;
-      for (uint64_t d = 0; d < dim; d++) {
-        dif[d] = h0[j * dim + d] - t0[(uint64_t)k * dim + d];
-        d2 = d2 + dif[d] * dif[d];
-      }
-      float gc = 0.0f;
-      if (d2 > 0.0f) {
-        const float pb = powf(d2, b);
-        gc = (m2ab * pb) / d2;
-        gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
-      }
-      for (uint64_t d = 0; d < dim; d++) h[d] = h0[j * dim + d] + clip4(gc * dif[d]) * alpha;
       const uint64_t ik = orc_rng_edge_key(ekey, i);
-      for (uint64_t p = 0; p < n_neg; p++) {
-        const uint32_t kk = orc_rng_index(ik, p, n_vertices);
-        if (kk == j) continue;
-        float dd = 0.0f;
+      if (generic) {
+        float g1[16], g2v[16];
+        const float dist = euclid_dist_grad(h0 + j * dim, t0 + (uint64_t)k * dim, dim, g1);
+        const float gc = generic_attract_coeff(dist, a, b);
+        for (uint64_t d = 0; d < dim; d++) h[d] = h0[j * dim + d] + clip4(gc * g1[d]) * alpha;
+        for (uint64_t p = 0; p < n_neg; p++) {
+          const uint32_t kk = orc_rng_index(ik, p, n_vertices);
+          if (kk == j) continue;
+          const float dd = euclid_dist_grad(h, t0 + (uint64_t)kk * dim, dim, g1);
+          const float gcn = generic_repel_coeff(dd, a, b, gamma);
+          for (uint64_t d = 0; d < dim; d++) h[d] = h[d] + clip4(gcn * g1[d]) * alpha;
+        }
+        for (uint64_t d = 0; d < dim; d++) acc[d] = acc[d] + (h[d] - h0[j * dim + d]);
+        /* twin (k,j): current = h0[k], other = t0[j]; the other side moves by
+         * clip(gc * grad_other) * alpha, grad_other from distance(other, current) */
+        const float dk = euclid_dist_grad(h0 + (uint64_t)k * dim, t0 + j * dim, dim, g1);
+        (void)euclid_dist_grad(t0 + j * dim, h0 + (uint64_t)k * dim, dim, g2v);
+        const float gck = generic_attract_coeff(dk, a, b);
+        for (uint64_t d = 0; d < dim; d++) t[d] = t[d] + clip4(gck * g2v[d]) * alpha;
+      } else {
+        float d2 = 0.0f;
         for (uint64_t d = 0; d < dim; d++) {
-          dif[d] = h[d] - t0[(uint64_t)kk * dim + d];
-          dd = dd + dif[d] * dif[d];
+          dif[d] = h0[j * dim + d] - t0[(uint64_t)k * dim + d];
+          d2 = d2 + dif[d] * dif[d];
         }
-        float g2 = 0.0f;
-        if (dd > 0.0f) {
-          const float pb = powf(dd, b);
-          g2 = g2b / ((0.001f + dd) * ((a * pb) * (g_ref_den ? dd : 1.0f) + 1.0f));
+        float gc = 0.0f;
+        if (d2 > 0.0f) {
+          const float pb = powf(d2, b);
+          gc = (m2ab * pb) / d2;
+          gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
         }
-        if (g2 > 0.0f)
-          for (uint64_t d = 0; d < dim; d++) h[d] = h[d] + clip4(g2 * dif[d]) * alpha;
+        for (uint64_t d = 0; d < dim; d++) h[d] = h0[j * dim + d] + clip4(gc * dif[d]) * alpha;
+        for (uint64_t p = 0; p < n_neg; p++) {
+          const uint32_t kk = orc_rng_index(ik, p, n_vertices);
+          if (kk == j) continue;
+          float dd = 0.0f;
+          for (uint64_t d = 0; d < dim; d++) {
+            dif[d] = h[d] - t0[(uint64_t)kk * dim + d];
+            dd = dd + dif[d] * dif[d];
+          }
+          float g2 = 0.0f;
+          if (dd > 0.0f) {
+            const float pb = powf(dd, b);
+            g2 = g2b / ((0.001f + dd) * ((a * pb) * (g_ref_den ? dd : 1.0f) + 1.0f));
+          }
+          if (g2 > 0.0f)
+            for (uint64_t d = 0; d < dim; d++) h[d] = h[d] + clip4(g2 * dif[d]) * alpha;
+        }
+        for (uint64_t d = 0; d < dim; d++) acc[d] = acc[d] + (h[d] - h0[j * dim + d]);
+        d2 = 0.0f;
+        for (uint64_t d = 0; d < dim; d++) {
+          dif[d] = h0[(uint64_t)k * dim + d] - t0[j * dim + d];
+          d2 = d2 + dif[d] * dif[d];
+        }
+        gc = 0.0f;
+        if (d2 > 0.0f) {
+          const float pb = powf(d2, b);
+          gc = (m2ab * pb) / d2;
+          gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
+        }
+        for (uint64_t d = 0; d < dim; d++) t[d] = t[d] + (-(clip4(gc * dif[d]) * alpha));
       }
-      for (uint64_t d = 0; d < dim; d++) acc[d] = acc[d] + (h[d] - h0[j * dim + d]);
-      d2 = 0.0f;
-      for (uint64_t d = 0; d < dim; d++) {
-        dif[d] = h0[(uint64_t)k * dim + d] - t0[j * dim + d];
-        d2 = d2 + dif[d] * dif[d];
-      }
-      gc = 0.0f;
-      if (d2 > 0.0f) {
-        const float pb = powf(d2, b);
-        gc = (m2ab * pb) / d2;
-        gc = gc / ((a * pb) * (g_ref_den ? d2 : 1.0f) + 1.0f);
-      }
-      for (uint64_t d = 0; d < dim; d++) t[d] = t[d] + (-(clip4(gc * dif[d]) * alpha));
       np += 1;
       nn += n_neg;
     }
@@ -727,9 +765,19 @@ ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, ui
   }
 }
 
+ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, uint64_t dim, const uint32_t *head,
+                                const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices, const double *eps,
+                                double *eons, const double *epns, double *eonns, float a, float b, float gamma,
+                                float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts) {
+  sgd_epoch_sync(head_emb, tail_emb, shared, 0, dim, head, tail, n_edges, n_vertices, eps, eons, epns, eonns, a, b,
+                 gamma, alpha, epoch, seed, counts);
+}
+
 /* Optimizer::step_epochs (optimizer.rs:297-366): tail snapshot per call,
  * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild, 2 = the
- * build's synchronous row-owner epoch (orc_sgd_epoch_sync).
+ * build's synchronous row-owner epoch (orc_sgd_epoch_sync), 3 = that epoch with
+ * the generic-metric forms, 4 = the serial generic-metric epoch
+ * (optimize_layout_generic.rs:35-124, Euclidean metric).
  * semantics: bit 1 = one shared array instead of the tail snapshot, bit 2 =
  * umap-learn's force denominators (both opt-in departures; 0 = reference). */
 ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const uint32_t *head, const uint32_t *tail,
@@ -747,9 +795,12 @@ ORC_API void orc_step_epochs(float *embedding, uint64_t n, uint64_t dim, const u
     if (mode == 0)
       orc_sgd_epoch_serial(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
                            gamma, alpha, e, seed, counts);
-    else if (mode == 2)
-      orc_sgd_epoch_sync(embedding, copy, shared, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a,
-                         b, gamma, alpha, e, seed, counts);
+    else if (mode == 2 || mode == 3)
+      sgd_epoch_sync(embedding, copy, shared, mode == 3, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns,
+                     eonns, a, b, gamma, alpha, e, seed, counts);
+    else if (mode == 4)
+      sgd_epoch_generic_serial(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
+                               gamma, alpha, e, seed, counts);
     else
       orc_sgd_epoch_hogwild(embedding, copy, dim, head, tail, n_edges, (uint32_t)n, eps, eons, epns, eonns, a, b,
                             gamma, alpha, e, seed, counts);
@@ -772,11 +823,12 @@ static float euclid_dist_grad(const float *x, const float *y, uint64_t dim, floa
   return dist;
 }
 
-ORC_API void orc_sgd_epoch_generic_euclid(float *head_emb, float *tail_emb, uint64_t dim, const uint32_t *head,
-                                          const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices,
-                                          const double *eps, double *eons, const double *epns, double *eonns,
-                                          float a, float b, float gamma, float alpha, uint64_t epoch, uint64_t seed) {
+static void sgd_epoch_generic_serial(float *head_emb, float *tail_emb, uint64_t dim, const uint32_t *head,
+                                     const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices, const double *eps,
+                                     double *eons, const double *epns, double *eonns, float a, float b, float gamma,
+                                     float alpha, uint64_t epoch, uint64_t seed, uint64_t *counts) {
   uint64_t ekey = orc_rng_epoch_key(seed, epoch);
+  uint64_t np = 0, nn = 0;
   float *g1 = (float *)malloc(dim * 4), *g2 = (float *)malloc(dim * 4);
   for (uint64_t i = 0; i < n_edges; i++) {
     if (!(eons[i] <= (double)epoch)) continue;
@@ -808,9 +860,23 @@ ORC_API void orc_sgd_epoch_generic_euclid(float *head_emb, float *tail_emb, uint
       }
     }
     eonns[i] = eonns[i] + (double)n_neg * epns[i];
+    np += 1;
+    nn += n_neg;
   }
   free(g1);
   free(g2);
+  if (counts) {
+    counts[0] += np;
+    counts[1] += nn;
+  }
+}
+
+ORC_API void orc_sgd_epoch_generic_euclid(float *head_emb, float *tail_emb, uint64_t dim, const uint32_t *head,
+                                          const uint32_t *tail, uint64_t n_edges, uint32_t n_vertices,
+                                          const double *eps, double *eons, const double *epns, double *eonns,
+                                          float a, float b, float gamma, float alpha, uint64_t epoch, uint64_t seed) {
+  sgd_epoch_generic_serial(head_emb, tail_emb, dim, head, tail, n_edges, n_vertices, eps, eons, epns, eonns, a, b,
+                           gamma, alpha, epoch, seed, NULL);
 }
 
 ORC_API int32_t orc_num_threads(void) {

@@ -167,16 +167,21 @@ def test_chunked_resume_equals_uninterrupted_serial(gpu, oracle):
 
 
 def test_generic_metric_host_path_matches_restatement(gpu, oracle):
-    """MetricType::Generic runs the reference's serial generic loop with the
-    caller's Metric::distance; with the Euclidean metric as the callback it
-    equals the restatement of optimize_layout_generic.rs."""
+    """MetricType::Generic with a user metric runs the reference's serial
+    generic loop with the caller's Metric::distance; with Euclidean distances
+    in the callback it equals the restatement of optimize_layout_generic.rs.
+    (The library's own EuclideanMetric runs on the device instead.)"""
     _, idx, d, init = ring_test_data(seed=7, n=120, k=10)
     cfg = gpu.UmapConfig()
     cfg.graph.n_neighbors = 10
     m = gpu.Umap(cfg).learn_manifold(np.zeros((120, 1)), idx, d)
     opt = gpu.Optimizer.new(m, init, 6, cfg, gpu.MetricType.Generic, seed=2)
     ost = oracle.OptimizerState(oracle.learn_manifold(idx, d), init, 6, seed=2)
-    metric = gpu.EuclideanMetric()
+
+    class UserEuclidean(gpu.EuclideanMetric):
+        pass
+
+    metric = UserEuclidean()
     opt.step_epochs(3, metric)
     ost.step_epochs_generic_euclid(3)
     assert np.allclose(opt.embedding(), ost.embedding, atol=1e-5)

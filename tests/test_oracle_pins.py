@@ -507,3 +507,135 @@ def test_sync_reaches_the_hogwild_quality(oracle):
             st.step_epochs(200, mode)
             ce[mode] = E.fuzzy_cross_entropy(m.indptr, m.indices, m.data, st.embedding, m.a, m.b)
         assert abs(ce["sync"] - ce["hogwild"]) / ce["hogwild"] < 0.03, (seed, ce)
+
+
+# ----------------------------------------------- generic-metric epoch (Euclidean)
+# optimize_layout_generic.rs:35-124 with EuclideanMetric::distance
+# (distances.rs:22-38): d = sqrt(sum diff^2), grad = diff / (d + 1e-6).
+def _generic_known(oracle, mode):
+    a, b, gamma, alpha = F(1.2), F(0.95), F(1.3), F(1.0) * (F(1.0) - F(1.0) / F(10.0))
+    ip = np.array([0, 1, 2, 2], np.uint64)
+    ix = np.array([1, 0], np.uint32)
+    m = oracle.Manifold(ip, ix, np.ones(2, F), np.ones(3, F), np.zeros(3, F), 3, a, b)
+    init = np.array([[0.0, 0.0], [3.0, 4.0], [10.0, 10.0]], F)
+    st = oracle.OptimizerState(m, init, 10, negative_sample_rate=0, seed=3, repulsion_strength=float(gamma))
+    e0 = st.embedding.copy()  # normalised to [0, 10] by Optimizer::new
+    st.step_epochs(2, mode)  # epoch 0 samples nothing; epoch 1 runs both edges once
+    return st, e0, a, b, alpha
+
+
+def _generic_step(h, t, a, b, alpha):
+    """One attraction step of the generic loop in float64: both ends."""
+    diff = h - t
+    d = math.sqrt((diff ** 2).sum())
+    w = 1.0 / (1.0 + float(a) * d ** (2.0 * float(b)))
+    gc = 2.0 * float(b) * (w - 1.0) / (d + 1e-6)
+    g = diff / (d + 1e-6)
+    return np.clip(gc * g, -4, 4) * float(alpha), np.clip(gc * -g, -4, 4) * float(alpha)
+
+
+def test_generic_attraction_known_answer(oracle):
+    """Edge (0,1) then its twin (1,0), no negatives: serial replay in float64.
+    The forms differ from the Euclidean path's (d, not d^2, in the curve and a
+    1/(d+1e-6) factor), so a mix-up of the two paths fails here."""
+    st, e0, a, b, alpha = _generic_known(oracle, "generic_serial")
+    h = e0.astype(np.float64)
+    t = e0.astype(np.float64).copy()
+    dh, dt = _generic_step(h[0], t[1], a, b, alpha)
+    h[0] += dh
+    t[1] += dt
+    dh, dt = _generic_step(h[1], t[0], a, b, alpha)
+    h[1] += dh
+    t[0] += dt
+    assert np.allclose(st.embedding[:2], h[:2], rtol=1e-5, atol=1e-5)
+    assert np.array_equal(st.embedding[2], e0[2])
+    assert st.counts.tolist() == [2, 0]
+    # the Euclidean path moves the same pair differently
+    eu, _, _, _, _ = _generic_known(oracle, "serial")
+    assert not np.allclose(eu.embedding[:2], st.embedding[:2], rtol=1e-3)
+
+
+def test_generic_repulsion_known_answer(oracle):
+    """One edge with one negative sample per epoch, replayed in float64 with
+    the repulsion coefficient (2 gamma) b w / (d + 1e-6)."""
+    a, b, gamma, alpha = F(1.2), F(0.95), F(1.3), F(0.5)
+    H = np.array([[0.0, 0.0], [0.3, 0.4], [1.0, 2.0]], F)
+    T = H.copy()
+    head, tail = np.array([0], np.uint32), np.array([1], np.uint32)
+    eps, eons = np.array([1.0]), np.array([1.0])
+    epns, eonns = np.array([1.0]), np.array([0.0])  # n_neg = (1-0)/1 = 1
+    k = oracle.rng_index(0, 1, 0, 0, 3)
+    H0 = H.copy()
+    p = oracle._ptr
+    oracle.lib().orc_sgd_epoch_generic_euclid(p(H), p(T), 2, p(head), p(tail), 1, 3, p(eps), p(eons), p(epns),
+                                              p(eonns), a, b, gamma, alpha, 1, 0)
+    h = H0[0].astype(np.float64)
+    t = H0.astype(np.float64).copy()
+    dh, dt = _generic_step(h, t[1], a, b, alpha)
+    h = h + dh
+    t[1] += dt
+    if k != 0:
+        diff = h - t[k]
+        d = math.sqrt((diff ** 2).sum())
+        w = 1.0 / (1.0 + float(a) * d ** (2.0 * float(b)))
+        gc = 2.0 * float(gamma) * float(b) * w / (d + 1e-6)
+        h = h + np.clip(gc * diff / (d + 1e-6), -4, 4) * float(alpha)
+    assert np.allclose(H[0], h, rtol=1e-5)
+    assert np.allclose(T[1], t[1], rtol=1e-5)
+    assert eonns[0] == 1.0
+
+
+def test_generic_sync_equals_serial_on_a_matching_without_negatives(oracle):
+    """The identity of the Euclidean synchronous epoch holds for the generic
+    forms too: one edge per row, no negatives, so the synchronous generic
+    epoch equals the serial generic loop bit for bit."""
+    m, init = _matching_state(oracle, 200, 13, 0)
+    ser = oracle.OptimizerState(m, init, 30, negative_sample_rate=0, seed=4)
+    syn = oracle.OptimizerState(m, init, 30, negative_sample_rate=0, seed=4)
+    ser.step_epochs(12, "generic_serial")
+    syn.step_epochs(12, "generic_sync")
+    assert np.array_equal(ser.embedding.view(np.uint32), syn.embedding.view(np.uint32))
+    assert ser.counts.tolist() == syn.counts.tolist()
+    # and the generic loop is not the Euclidean one
+    eu = oracle.OptimizerState(m, init, 30, negative_sample_rate=0, seed=4)
+    eu.step_epochs(12, "serial")
+    assert not np.array_equal(eu.embedding, ser.embedding)
+
+
+def test_generic_sync_is_jacobi_within_a_row(oracle):
+    """Row 0 with two neighbours: the synchronous generic epoch adds both
+    attraction steps computed from the epoch-start row."""
+    n = 3
+    ip = np.array([0, 2, 3, 4], np.uint64)
+    ix = np.array([1, 2, 0, 0], np.uint32)
+    m = oracle.Manifold(ip, ix, np.ones(4, F), np.ones(n, F), np.zeros(n, F), n, F(1.5), F(0.9))
+    init = np.array([[0.0, 0.0], [1.0, 0.5], [-0.5, 2.0]], F)
+    syn = oracle.OptimizerState(m, init, 20, negative_sample_rate=0, seed=1)
+    h0 = syn.embedding.copy().astype(np.float64)
+    syn.step_epochs(2, "generic_sync")
+    alpha = F(1.0) * (F(1.0) - F(1.0) / F(20.0))
+    g1, _ = _generic_step(h0[0], h0[1], F(1.5), F(0.9), alpha)
+    g2, _ = _generic_step(h0[0], h0[2], F(1.5), F(0.9), alpha)
+    assert np.allclose(syn.embedding[0], h0[0] + g1 + g2, rtol=1e-5, atol=1e-5)
+    # rows 1 and 2: own step toward 0 (their only edge)
+    for r in (1, 2):
+        g, _ = _generic_step(h0[r], h0[0], F(1.5), F(0.9), alpha)
+        assert np.allclose(syn.embedding[r], h0[r] + g, rtol=1e-5, atol=1e-5)
+
+
+def test_generic_sync_reaches_the_serial_quality(oracle):
+    """Statistical parity of the synchronous generic epoch with the serial
+    generic loop the reference runs: final fuzzy cross-entropy within 3%."""
+    import umap_eval as E
+    for seed in (3, 5):
+        x, _ = gmm(2000, 16, 5, seed=seed)
+        idx, d = knn_exact(x, 15)
+        m = oracle.learn_manifold(idx, d)
+        init = np.random.default_rng(seed).uniform(-10, 10, (2000, 2)).astype(F)
+        ce = {}
+        for mode in ("generic_serial", "generic_sync"):
+            st = oracle.OptimizerState(m, init, 200, seed=seed)
+            st.step_epochs(200, mode)
+            ce[mode] = E.fuzzy_cross_entropy(m.indptr, m.indices, m.data, st.embedding, m.a, m.b)
+        print(seed, ce)
+        assert abs(ce["generic_sync"] - ce["generic_serial"]) / ce["generic_serial"] < 0.03, (seed, ce)

@@ -661,6 +661,22 @@ umapgpu_status umapgpu_optimizer_step_epochs_generic(umapgpu_optimizer* o, uint6
   });
 }
 
+umapgpu_status umapgpu_optimizer_step_epochs_builtin(umapgpu_optimizer* o, uint64_t n, int32_t metric) {
+  return guarded([&] {
+    if (!o) throw Error(Err::InvalidArgument, "null handle");
+    if (metric != UMAPGPU_BUILTIN_METRIC_EUCLIDEAN) throw Error(Err::InvalidArgument, "unknown built-in metric");
+    check_overstep(o->st, n);
+    // optimizer.rs:319-361: the optimizer's MetricType picks the loop; the
+    // metric only matters to the generic one
+    const bool generic = o->st.metric_type == UMAPGPU_METRIC_GENERIC;
+    if (generic && o->st.mode != SgdMode::Serial && o->st.dim > 8)
+      throw Error(Err::Unsupported,
+                  "parallel generic-metric SGD on the device supports n_components <= 8 (SERIAL supports any)");
+    ensure_device();
+    optimizer_step(&o->st, n, stream(), generic);
+  });
+}
+
 umapgpu_status umapgpu_optimizer_set_mode(umapgpu_optimizer* o, int32_t mode) {
   return guarded([&] {
     if (mode < 0 || mode > 5) panic("unknown SGD mode");

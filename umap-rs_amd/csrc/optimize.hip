@@ -174,6 +174,31 @@ __device__ inline float repel_coeff(const P& A, float d2, float pb) {
   return A.g2b / ((0.001f + d2) * ((A.a * pb) * (A.ref_den ? d2 : 1.0f) + 1.0f));
 }
 
+// Which force forms an epoch runs.  kMetricEuclidean: the Euclidean
+// specialization (optimize_layout_euclidean.rs, squared distances).
+// kMetricGenericEuclid: the generic-metric loop (optimize_layout_generic.rs)
+// with the Euclidean metric's distance and gradient (distances.rs:22-38):
+//   d = sqrt(sum diff^2), gradient = diff / (d + 1e-6),
+//   w = 1 / (1 + a d^(2b))  (1 when d = 0; the reference's powi(.., -1)),
+//   attraction (2b)(w - 1) / (d + 1e-6), repulsion ((2 gamma) b) w / (d + 1e-6)
+// (:68-74 and :108-114).  The reverse gradient of the tail side is exactly
+// the negated gradient, so the tail moves by exactly -clip(gc * grad) * alpha.
+constexpr int kMetricEuclidean = 0;
+constexpr int kMetricGenericEuclid = 1;
+
+template <typename P>
+__device__ inline float generic_w(const P& A, float dist, const LibmTables& tb) {
+  return (dist > 0.0f) ? 1.0f / (1.0f + A.a * libm_powf(dist, 2.0f * A.b, tb)) : 1.0f;
+}
+template <typename P>
+__device__ inline float generic_attract_coeff(const P& A, float dist, float den, const LibmTables& tb) {
+  return ((2.0f * A.b) * (generic_w(A, dist, tb) - 1.0f)) / den;
+}
+template <typename P>
+__device__ inline float generic_repel_coeff(const P& A, float dist, float den, const LibmTables& tb) {
+  return (A.g2b * generic_w(A, dist, tb)) / den;
+}
+
 // One lane = one head row.  D = embedding dimension (registers).
 template <int D, typename NextT, bool ATOMIC>
 __global__ __launch_bounds__(256) void sgd_rows_kernel(float* __restrict__ H, float* __restrict__ T,
@@ -324,16 +349,18 @@ __global__ __launch_bounds__(256) void sgd_rows_dyn_kernel(float* __restrict__ H
 }
 
 // Deterministic serial mode: the reference's edge order on one lane
-// (optimize_layout_euclidean.rs:168-231 executed in order).  H and T may be
-// the same array (shared-buffer semantics), so they carry no restrict.  A
-// sweep runs its fits one after another, each with its own parameters.
-template <typename NextT, bool SWEEP>
+// (optimize_layout_euclidean.rs:168-231 executed in order, or the generic
+// loop optimize_layout_generic.rs:55-123 for kMetricGenericEuclid).  H and T
+// may be the same array (shared-buffer semantics), so they carry no restrict.
+// A sweep runs its fits one after another, each with its own parameters.
+template <typename NextT, bool SWEEP, int METRIC>
 __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64_t* __restrict__ row_ptr,
                                   const uint32_t* __restrict__ tail, const double2* __restrict__ rate,
                                   double2* __restrict__ sched, NextT* __restrict__ next, SgdArgs A,
                                   const SweepFit* __restrict__ fits, const SweepEpoch* __restrict__ fit_epoch,
                                   uint32_t nfits, unsigned long long* __restrict__ stats) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const LibmTables tb = libm_default_tables();
   uint64_t npos = 0, nneg = 0;
   uint32_t f = 0;
   for (uint32_t j = 0; j < A.n; ++j) {
@@ -359,13 +386,25 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64
         const float df = h[d] - t[d];
         d2 = d2 + df * df;
       }
-      float gc = 0.0f;
-      if (d2 > 0.0f) gc = attract_coeff(F, d2, libm_powf(d2, F.b));
-      for (uint32_t d = 0; d < dim; ++d) {
-        const float df = h[d] - t[d];
-        const float g = clip4(gc * df);
-        h[d] = h[d] + g * F.alpha;
-        t[d] = t[d] - g * F.alpha;
+      if constexpr (METRIC == kMetricGenericEuclid) {
+        // both gradients come from the values before this edge's update;
+        // coordinate d's update touches coordinate d only
+        const float dist = sqrtf(d2), den = dist + 1e-6f;
+        const float gc = generic_attract_coeff(F, dist, den, tb);
+        for (uint32_t d = 0; d < dim; ++d) {
+          const float g1 = (h[d] - t[d]) / den, g2 = (t[d] - h[d]) / den;
+          h[d] = h[d] + clip4(gc * g1) * F.alpha;
+          t[d] = t[d] + clip4(gc * g2) * F.alpha;
+        }
+      } else {
+        float gc = 0.0f;
+        if (d2 > 0.0f) gc = attract_coeff(F, d2, libm_powf(d2, F.b, tb));
+        for (uint32_t d = 0; d < dim; ++d) {
+          const float df = h[d] - t[d];
+          const float g = clip4(gc * df);
+          h[d] = h[d] + g * F.alpha;
+          t[d] = t[d] - g * F.alpha;
+        }
       }
       const double2 rt = rate[e];
       sc.x = sc.x + rt.x;
@@ -380,13 +419,19 @@ __global__ void sgd_serial_kernel(float* H, float* T, uint32_t dim, const uint64
           const float df = h[d] - o[d];
           dd = dd + df * df;
         }
-        float g2 = 0.0f;
-        if (dd > 0.0f) g2 = repel_coeff(F, dd, libm_powf(dd, F.b));
-        if (g2 > 0.0f)
-          for (uint32_t d = 0; d < dim; ++d) {
-            const float df = h[d] - o[d];
-            h[d] = h[d] + clip4(g2 * df) * F.alpha;
-          }
+        if constexpr (METRIC == kMetricGenericEuclid) {
+          const float dist = sqrtf(dd), den = dist + 1e-6f;
+          const float gcn = generic_repel_coeff(F, dist, den, tb);
+          for (uint32_t d = 0; d < dim; ++d) h[d] = h[d] + clip4(gcn * ((h[d] - o[d]) / den)) * F.alpha;
+        } else {
+          float g2 = 0.0f;
+          if (dd > 0.0f) g2 = repel_coeff(F, dd, libm_powf(dd, F.b, tb));
+          if (g2 > 0.0f)
+            for (uint32_t d = 0; d < dim; ++d) {
+              const float df = h[d] - o[d];
+              h[d] = h[d] + clip4(g2 * df) * F.alpha;
+            }
+        }
       }
       sc.y = sc.y + (double)nn * rt.y;
       sched[e] = sc;
@@ -784,7 +829,7 @@ __device__ inline void finish_row(float* H, float* T, uint32_t j, const float* h
   }
 }
 
-template <int D, typename NextT, bool SHARED, bool TWIN, int BT, bool SWEEP>
+template <int D, typename NextT, bool SHARED, bool TWIN, int BT, bool SWEEP, int METRIC>
 __global__ __launch_bounds__(BT) void sgd_fused_kernel(
     float* H, float* T, const float* __restrict__ HT0, const float* __restrict__ T0,
     const uint64_t* __restrict__ row_ptr, const uint32_t* __restrict__ first_row,
@@ -965,20 +1010,26 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
               hk[d] = 0.0f;
             }
           }
-          {  // own attractive sample (:278-306)
+          {  // own attractive sample (:278-306; generic forms optimize_layout_generic.rs:64-88)
             float d2 = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
               diff[d] = h0[d] - tk[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
             float ga[D];
+            if constexpr (METRIC == kMetricGenericEuclid) {
+              const float dist = sqrtf(d2), den = dist + 1e-6f;
+              const float gc = generic_attract_coeff(FL, dist, den, tb);
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-              ga[d] = clip4(gc * diff[d]) * FL.alpha;
-              h[d] = h0[d] + ga[d];
+              for (int d = 0; d < D; ++d) ga[d] = clip4(gc * (diff[d] / den)) * FL.alpha;
+            } else {
+              const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
+#pragma unroll
+              for (int d = 0; d < D; ++d) ga[d] = clip4(gc * diff[d]) * FL.alpha;
             }
+#pragma unroll
+            for (int d = 0; d < D; ++d) h[d] = h0[d] + ga[d];
             if (!TWIN) tail_sub<D>(T + (uint64_t)k * D, tk, ga);  // this edge's tail side, now
           }
           for (uint32_t p = 0; p < nn; ++p) {  // repulsive samples (:313-341)
@@ -1005,10 +1056,17 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
               diff[d] = h[d] - o[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float g2 = (d2 > 0.0f) ? repel_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
-            if (g2 > 0.0f) {
+            if constexpr (METRIC == kMetricGenericEuclid) {  // optimize_layout_generic.rs:97-118
+              const float dist = sqrtf(d2), den = dist + 1e-6f;
+              const float gcn = generic_repel_coeff(FL, dist, den, tb);
 #pragma unroll
-              for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * FL.alpha;
+              for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(gcn * (diff[d] / den)) * FL.alpha;
+            } else {
+              const float g2 = (d2 > 0.0f) ? repel_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
+              if (g2 > 0.0f) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) h[d] = h[d] + clip4(g2 * diff[d]) * FL.alpha;
+              }
             }
           }
           float dt[D];
@@ -1021,9 +1079,16 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
               diff[d] = hk[d] - t0[d];
               d2 = d2 + diff[d] * diff[d];
             }
-            const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
+            if constexpr (METRIC == kMetricGenericEuclid) {
+              const float dist = sqrtf(d2), den = dist + 1e-6f;
+              const float gc = generic_attract_coeff(FL, dist, den, tb);
 #pragma unroll
-            for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * FL.alpha);
+              for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * (diff[d] / den)) * FL.alpha);
+            } else {
+              const float gc = (d2 > 0.0f) ? attract_coeff(FL, d2, libm_powf(d2, FL.b, tb)) : 0.0f;
+#pragma unroll
+              for (int d = 0; d < D; ++d) dt[d] = -(clip4(gc * diff[d]) * FL.alpha);
+            }
           }
 #pragma unroll
           for (int d = 0; d < D; ++d) {
@@ -1211,14 +1276,37 @@ void launch_rows(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   UMAP_LAUNCH_CHECK();
 }
 
-template <typename NextT, bool TWIN>
+template <typename NextT, bool TWIN, int METRIC>
 void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s);
+
+// One epoch of the generic-metric loop with the Euclidean metric: SERIAL runs
+// the reference's order on one lane; every other mode runs the row-owner
+// epoch (synchronous on twin-symmetric schedules, tail sides by
+// compare-and-swap otherwise).  The per-edge forms are generic_*_coeff.
+template <typename NextT>
+void launch_generic_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
+  if (st->mode == SgdMode::Serial) {
+    sgd_serial_kernel<NextT, false, kMetricGenericEuclid><<<1, 64, 0, s>>>(
+        st->head_emb.as<float>(), tail_buffer(st), (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
+        st->tail.as<uint32_t>(), st->rate.as<double2>(), st->sched.as<double2>(), st->next.as<NextT>(), A, nullptr,
+        nullptr, 0u, st->stats.as<unsigned long long>());
+    UMAP_LAUNCH_CHECK();
+    return;
+  }
+  if (st->dim > 8)
+    throw Error(Err::Unsupported,
+                "parallel generic-metric SGD on the device supports n_components <= 8 (SERIAL supports any)");
+  if (st->twin_ok && st->mode != SgdMode::HogwildWindowAtomic)
+    launch_owner<NextT, true, kMetricGenericEuclid>(st, A, s);
+  else
+    launch_owner<NextT, false, kMetricGenericEuclid>(st, A, s);
+}
 
 template <typename NextT>
 void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   switch (st->mode) {
     case SgdMode::Serial:
-      sgd_serial_kernel<NextT, false><<<1, 64, 0, s>>>(st->head_emb.as<float>(), tail_buffer(st),
+      sgd_serial_kernel<NextT, false, kMetricEuclidean><<<1, 64, 0, s>>>(st->head_emb.as<float>(), tail_buffer(st),
                                                        (uint32_t)st->dim, st->row_ptr.as<uint64_t>(),
                                                        st->tail.as<uint32_t>(), st->rate.as<double2>(),
                                                        st->sched.as<double2>(), st->next.as<NextT>(), A, nullptr,
@@ -1242,15 +1330,15 @@ void launch_epoch(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
       break;
     case SgdMode::HogwildWindowAtomic:
       if (st->dim <= 8)
-        launch_owner<NextT, false>(st, A, s);
+        launch_owner<NextT, false, kMetricEuclidean>(st, A, s);
       else
         launch_rows<NextT, true>(st, A, s);
       break;
     default:
       if (st->dim <= 8 && st->twin_ok)
-        launch_owner<NextT, true>(st, A, s);
+        launch_owner<NextT, true, kMetricEuclidean>(st, A, s);
       else if (st->dim <= 8)
-        launch_owner<NextT, false>(st, A, s);
+        launch_owner<NextT, false, kMetricEuclidean>(st, A, s);
       else
         launch_rows<NextT, true>(st, A, s);
   }
@@ -1390,7 +1478,7 @@ struct OwnerEpoch {
   unsigned long long* stats;
 };
 
-template <typename NextT, bool TWIN, bool SWEEP>
+template <typename NextT, bool TWIN, bool SWEEP, int METRIC>
 void run_owner_epoch(const OwnerEpoch& o, const SgdArgs& A, hipStream_t s) {
   const float* HT0 = nullptr;
   const float* T0 = nullptr;
@@ -1410,11 +1498,13 @@ void run_owner_epoch(const OwnerEpoch& o, const SgdArgs& A, hipStream_t s) {
 #define UMAP_FUSED_CASE(DD)                                                                                       \
   case DD:                                                                                                        \
     if (o.shared)                                                                                                 \
-      launch_fused(sgd_fused_kernel<DD, NextT, true, TWIN, kFusedBlock, SWEEP>, o.n_windows, s, o.H, o.T, HT0, T0, \
+      launch_fused(sgd_fused_kernel<DD, NextT, true, TWIN, kFusedBlock, SWEEP, METRIC>, o.n_windows, s, o.H, o.T,  \
+                   HT0, T0,                                                                                       \
                    o.row_ptr, o.first_row, o.n_windows, o.tail, o.rate, o.sched, nx, A, o.fits, o.fit_epoch,      \
                    o.nfits, o.stats);                                                                             \
     else                                                                                                          \
-      launch_fused(sgd_fused_kernel<DD, NextT, false, TWIN, kFusedBlock, SWEEP>, o.n_windows, s, o.H, o.T, HT0,   \
+      launch_fused(sgd_fused_kernel<DD, NextT, false, TWIN, kFusedBlock, SWEEP, METRIC>, o.n_windows, s, o.H,     \
+                   o.T, HT0,                                                                                      \
                    T0, o.row_ptr, o.first_row, o.n_windows, o.tail, o.rate, o.sched, nx, A, o.fits, o.fit_epoch,  \
                    o.nfits, o.stats);                                                                             \
     break;
@@ -1434,7 +1524,7 @@ void run_owner_epoch(const OwnerEpoch& o, const SgdArgs& A, hipStream_t s) {
   UMAP_LAUNCH_CHECK();
 }
 
-template <typename NextT, bool TWIN>
+template <typename NextT, bool TWIN, int METRIC>
 void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   OwnerEpoch o{};
   o.n = st->n;
@@ -1452,7 +1542,7 @@ void launch_owner(OptimizerState* st, const SgdArgs& A, hipStream_t s) {
   o.sched = st->sched.as<double2>();
   o.next = st->next.p;
   o.stats = st->stats.as<unsigned long long>();
-  run_owner_epoch<NextT, TWIN, false>(o, A, s);
+  run_owner_epoch<NextT, TWIN, false, METRIC>(o, A, s);
 }
 
 }  // namespace
@@ -1609,7 +1699,7 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
   UMAP_HIP(hipStreamSynchronize(s));
 }
 
-void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
+void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s, bool generic_euclid) {
   const uint64_t start = st->current_epoch, end = st->current_epoch + n_epochs;
   if (st->n * st->dim && !(st->semantics & kSemanticsSharedBuffer))
     UMAP_HIP(hipMemcpyAsync(st->tail_emb.p, st->head_emb.p, st->n * st->dim * 4, hipMemcpyDeviceToDevice, s));
@@ -1633,7 +1723,11 @@ void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s) {
       A.epoch = e;
       A.epoch_f64 = (double)e;
       A.epoch_key = rng_epoch_key(st->seed, e);
-      if (st->next16)
+      if (generic_euclid && st->next16)
+        launch_generic_epoch<uint16_t>(st, A, s);
+      else if (generic_euclid)
+        launch_generic_epoch<uint32_t>(st, A, s);
+      else if (st->next16)
         launch_epoch<uint16_t>(st, A, s);
       else
         launch_epoch<uint32_t>(st, A, s);
@@ -1703,7 +1797,7 @@ void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t 
   float* T = (sw->semantics & kSemanticsSharedBuffer) ? H : sw->tail_emb.as<float>();
   unsigned long long* stt = sw->stats.as<unsigned long long>();
   if (sw->mode == SgdMode::Serial) {
-    sgd_serial_kernel<NextT, true><<<1, 64, 0, s>>>(H, T, (uint32_t)sw->dim, sw->row_ptr.as<uint64_t>(),
+    sgd_serial_kernel<NextT, true, kMetricEuclidean><<<1, 64, 0, s>>>(H, T, (uint32_t)sw->dim, sw->row_ptr.as<uint64_t>(),
                                                      sw->tail.as<uint32_t>(), sw->rate.as<double2>(),
                                                      sw->sched.as<double2>(), sw->next.as<NextT>(), A,
                                                      sw->fits.as<SweepFit>(), fe, nf, stt);
@@ -1731,9 +1825,9 @@ void sweep_epoch(SweepState* sw, uint64_t epoch, const SweepEpoch* fe, uint32_t 
   o.nfits = nf;
   o.stats = stt;
   if (sw->twin_ok)
-    run_owner_epoch<NextT, true, true>(o, A, s);
+    run_owner_epoch<NextT, true, true, kMetricEuclidean>(o, A, s);
   else
-    run_owner_epoch<NextT, false, true>(o, A, s);
+    run_owner_epoch<NextT, false, true, kMetricEuclidean>(o, A, s);
 }
 
 // Autotuner candidates: edges per row-owner window.

@@ -101,9 +101,11 @@ void optimizer_import(uint64_t n, uint64_t dim, uint64_t n_edges, const uint32_t
                       const double* eps, const double* eons, const double* epns, const double* eonns,
                       const float* embedding, OptimizerState* st, hipStream_t s);
 
-// Euclidean step_epochs body (optimizer.rs:306-365).  Overstep is checked by
-// the caller.
-void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s);
+// step_epochs body (optimizer.rs:306-365): the Euclidean specialization, or
+// with generic_euclid the generic-metric loop with the Euclidean metric
+// (optimize_layout_generic.rs:35-124 with distances.rs:22-38).  Overstep is
+// checked by the caller.
+void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s, bool generic_euclid = false);
 
 void optimizer_export(const OptimizerState& st, uint32_t* head, uint32_t* tail, double* eps, double* eons,
                       double* epns, double* eonns, float* embedding, hipStream_t s);

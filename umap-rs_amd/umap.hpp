@@ -22,6 +22,7 @@
 #include <limits>
 #include <memory>
 #include <optional>
+#include <typeinfo>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -328,11 +329,18 @@ class Optimizer {  // optimizer.rs:28-403
     if (h_) umapgpu_optimizer_destroy(h_);
   }
 
-  // Optimizer::step_epochs (optimizer.rs:297-366)
+  // Optimizer::step_epochs (optimizer.rs:297-366).  MetricType::Generic with
+  // EuclideanMetric itself (not a subclass) runs the generic loop on the
+  // device with the metric built in (n_components <= 8); any other metric
+  // runs the reference's serial generic loop on the host through the callback.
   void step_epochs(size_t n, const Metric& output_metric) {
     int32_t mt = 0;
     check(umapgpu_optimizer_export_params(h(), nullptr, nullptr, nullptr, nullptr, nullptr, &mt, nullptr));
-    if (mt == UMAPGPU_METRIC_GENERIC)
+    uint64_t dim = 0;
+    check(umapgpu_optimizer_info(h(), nullptr, nullptr, &dim));
+    if (mt == UMAPGPU_METRIC_GENERIC && typeid(output_metric) == typeid(EuclideanMetric) && dim <= 8)
+      check(umapgpu_optimizer_step_epochs_builtin(h(), n, UMAPGPU_BUILTIN_METRIC_EUCLIDEAN));
+    else if (mt == UMAPGPU_METRIC_GENERIC)
       check(umapgpu_optimizer_step_epochs_generic(h(), n, &detail::metric_trampoline,
                                                   const_cast<Metric*>(&output_metric)));
     else

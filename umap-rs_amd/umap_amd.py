@@ -106,6 +106,7 @@ def lib():
             "umapgpu_optimizer_new_device": ([P, P, u64, u64, u64, C.POINTER(_Config), i32, u64, C.POINTER(P)], i32),
             "umapgpu_optimizer_step_epochs": ([P, u64], i32),
             "umapgpu_optimizer_step_epochs_generic": ([P, u64, DISTANCE_FN, P], i32),
+            "umapgpu_optimizer_step_epochs_builtin": ([P, u64, i32], i32),
             "umapgpu_optimizer_set_mode": ([P, i32], i32),
             "umapgpu_optimizer_set_semantics": ([P, u32], i32),
             "umapgpu_optimizer_semantics": ([P], u32),
@@ -271,6 +272,9 @@ class UmapConfig:
 
 # ---------------------------------------------------------------------------
 # metric.rs / distances.rs
+
+BUILTIN_METRIC_EUCLIDEAN = 0  # umapgpu_optimizer_step_epochs_builtin: EuclideanMetric
+
 
 class MetricType(enum.IntEnum):
     """metric.rs:7-12"""
@@ -546,12 +550,19 @@ class Optimizer(_Handle):
         return Optimizer(out)
 
     def step_epochs(self, n: int, output_metric: Optional[Metric] = None) -> None:
-        """Optimizer::step_epochs (optimizer.rs:297-366)."""
+        """Optimizer::step_epochs (optimizer.rs:297-366).  MetricType::Generic with
+        this module's EuclideanMetric (the class itself, not a subclass) runs the
+        generic loop on the device with the metric built in (n_components <= 8);
+        any other metric runs the reference's serial generic loop on the host,
+        calling the metric's distance for every sample."""
         h = self._handle()
         mt = self.export_params()["metric_type"]
         if mt == MetricType.Generic:
             if output_metric is None:
                 raise UmapPanic("MetricType::Generic needs the output metric")
+            if type(output_metric) is EuclideanMetric and self.info()[2] <= 8:
+                _check(lib().umapgpu_optimizer_step_epochs_builtin(h, n, BUILTIN_METRIC_EUCLIDEAN))
+                return
             cb = _distance_callback(output_metric)
             _check(lib().umapgpu_optimizer_step_epochs_generic(h, n, cb, None))
         else:
