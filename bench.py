@@ -463,6 +463,50 @@ def large_graph_section(U, args):
             "sgd_roofline_frac": round(bpu * rate / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def generic_section(U, args, manifold, d_init, init, n, epochs, cfg):
+    """MetricType::Generic with the built-in EuclideanMetric on the same graph
+    (the generic loop's forms on the device) beside the reference's serial
+    generic loop on the host."""
+    m = manifold.clone()
+    U.synchronize()
+    opt = U.Optimizer.new_device(m, d_init.ptr, n, 2, epochs, cfg, U.MetricType.Generic, seed=2)
+    U.synchronize()
+    t = time.perf_counter()
+    opt.step_epochs(epochs, U.EuclideanMetric())
+    U.synchronize()
+    wall = time.perf_counter() - t
+    p, q, ms = opt.stats()
+    res = {"workload": f"same graph and init, MetricType::Generic + EuclideanMetric, {epochs} epochs, default mode",
+           "value": round(p / (ms / 1e3), 1), "unit": "edge-updates/s", "device_ms": round(ms, 2),
+           "wall_s": round(wall, 3), "updates": int(p), "negatives": int(q)}
+    if not args.skip_quality:
+        import umap_eval as E
+        g = opt.manifold().graph()
+        a, b = opt.manifold().curve_params()
+        res["final_fuzzy_ce"] = round(E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), opt.embedding(), a, b), 1)
+    log(f"generic metric: {ms:.1f} ms device for {epochs} epochs, {res['value']:.3g} updates/s")
+    if not args.skip_cpu:
+        import oracle as O
+        st = opt.manifold().to_state()
+        om = O.Manifold(st["indptr"], st["indices"], st["data"], st["sigmas"], st["rhos"], n, st["a"], st["b"])
+        ost = O.OptimizerState(om, init, epochs, seed=2)
+        ost.step_epochs(1, "generic_serial")  # epoch 0 samples nothing
+        c0 = int(ost.counts[0])
+        t = time.perf_counter()
+        done = 0
+        while done + 1 < epochs and time.perf_counter() - t < args.cpu_seconds / 3:
+            ost.step_epochs(1, "generic_serial")
+            done += 1
+        dt = time.perf_counter() - t
+        rate = (int(ost.counts[0]) - c0) / dt
+        res["cpu_baseline"] = {"value": round(rate, 1), "unit": "edge-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"the reference's serial generic loop (optimize_layout_generic.rs, "
+                                         f"Euclidean metric) restated in C, epochs 1..{done} of {epochs}, {dt:.1f}s"}
+        log(f"generic metric cpu baseline: {rate:.3g} updates/s on 1 thread")
+    del opt
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +523,7 @@ def main() -> int:
     ap.add_argument("--skip-quality", action="store_true")
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-init wall-to-target comparison")
     ap.add_argument("--skip-sweep", action="store_true", help="skip the batched-sweep comparison")
+    ap.add_argument("--skip-generic", action="store_true", help="skip the generic-metric (built-in Euclidean) run")
     ap.add_argument("--large-graph", action="store_true",
                     help="also run the 20M-point k=30 large-graph workload (graph build + SGD at ~1.2B edges)")
     ap.add_argument("--large-n", type=int, default=20_000_000)
@@ -658,6 +703,9 @@ def main() -> int:
                                "sample": f"same graph and init, Hogwild epochs 1..{done} of {epochs} "
                                          f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
         log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")
+
+    if not args.skip_generic:
+        out["generic_metric"] = generic_section(U, args, manifold, d_init, init, n, epochs, cfg)
 
     if not args.skip_sweep:
         out["sweep"] = sweep_section(U, args)
