@@ -285,6 +285,31 @@ UMAPGPU_API umapgpu_status umapgpu_fitted_from_host(umapgpu_manifold* manifold, 
                                                     uint64_t dim, const umapgpu_config* cfg, umapgpu_fitted** out);
 UMAPGPU_API void umapgpu_fitted_destroy(umapgpu_fitted* f);
 
+/* ---- checkpoints in the reference's bytes ------------------------------
+ * bincode::serialize / bincode::deserialize (bincode 1.3, the reference's
+ * dev-dependency, Cargo.toml:24) of the serde derives of LearnedManifold
+ * (manifold.rs:14-38), Optimizer (optimizer.rs:27-56) and FittedUmap
+ * (embedding.rs:438-443, with UmapConfig, config.rs:8-224), as used by
+ * tests.rs:132-215 and README.md:60-126.  *_to_bincode: with buf NULL,
+ * *len receives the byte count and nothing else happens; otherwise cap must
+ * be at least that count (else UMAPGPU_INVALID_ARGUMENT) and the bytes are
+ * written.  *_from_bincode: a malformed input (truncated, array version not
+ * 1, shape/data mismatch, bad bool/Option/enum tag, invalid CSR) gives
+ * UMAPGPU_INVALID_ARGUMENT with a "bincode: ..." message; trailing bytes
+ * are ignored, as bincode::deserialize does.  An Optimizer checkpoint holds
+ * no RNG state (the reference's negative samples come from an OS-seeded
+ * generator): `seed` keys the counter RNG of the restored optimizer. */
+UMAPGPU_API umapgpu_status umapgpu_manifold_to_bincode(const umapgpu_manifold* m, uint8_t* buf, uint64_t cap,
+                                                       uint64_t* len);
+UMAPGPU_API umapgpu_status umapgpu_manifold_from_bincode(const uint8_t* buf, uint64_t len, umapgpu_manifold** out);
+UMAPGPU_API umapgpu_status umapgpu_optimizer_to_bincode(const umapgpu_optimizer* opt, uint8_t* buf, uint64_t cap,
+                                                        uint64_t* len);
+UMAPGPU_API umapgpu_status umapgpu_optimizer_from_bincode(const uint8_t* buf, uint64_t len, uint64_t seed,
+                                                          umapgpu_optimizer** out);
+UMAPGPU_API umapgpu_status umapgpu_fitted_to_bincode(const umapgpu_fitted* f, uint8_t* buf, uint64_t cap,
+                                                     uint64_t* len);
+UMAPGPU_API umapgpu_status umapgpu_fitted_from_bincode(const uint8_t* buf, uint64_t len, umapgpu_fitted** out);
+
 /* ---- device memory and harness utilities (no reference counterpart) ----
  * Used by benchmarks and tests to keep inputs resident in HBM. */
 UMAPGPU_API umapgpu_status umapgpu_device_alloc(uint64_t bytes, void** ptr);

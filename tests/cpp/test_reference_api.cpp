@@ -119,35 +119,14 @@ static void test_checkpoint_serialization() {  // tests.rs:132-162
   EuclideanMetric metric;
   Optimizer opt(std::move(manifold), t.init, 50, config, metric.metric_type());
   opt.step_epochs(20, metric);
-  // serialise: every state array to host memory, then rebuild
-  uint64_t ne = 0, nv = 0, dim = 0;
-  check(umapgpu_optimizer_info(opt.handle(), &ne, &nv, &dim));
-  std::vector<uint32_t> head(ne), tail(ne);
-  std::vector<double> eps(ne), eons(ne), epns(ne), eonns(ne);
-  std::vector<float> emb(nv * dim);
-  check(umapgpu_optimizer_export_state(opt.handle(), head.data(), tail.data(), eps.data(), eons.data(), epns.data(),
-                                       eonns.data(), emb.data()));
-  uint64_t cur = 0, tot = 0, seed = 0;
-  float gamma = 0, ia = 0;
-  double nsr = 0;
-  int32_t mt = 0;
-  check(umapgpu_optimizer_export_params(opt.handle(), &cur, &tot, &gamma, &ia, &nsr, &mt, &seed));
-  SparseMat g = opt.manifold().graph();
-  auto sr = opt.manifold().sigmas_rhos();
-  auto ab = opt.manifold().curve_params();
-  umapgpu_manifold* m2 = nullptr;
-  check(umapgpu_manifold_from_host(g.n, g.nnz(), g.indptr.data(), g.indices.data(), g.data.data(), sr.first.data(),
-                                   sr.second.data(), ab.first, ab.second, &m2));
-  umapgpu_optimizer* o2 = nullptr;
-  check(umapgpu_optimizer_import_state(m2, ne, head.data(), tail.data(), eps.data(), eons.data(), epns.data(),
-                                       eonns.data(), emb.data(), dim, cur, tot, gamma, ia, nsr, mt, seed, &o2));
-  REQUIRE(umapgpu_optimizer_current_epoch(o2) == 20);
-  REQUIRE(umapgpu_optimizer_remaining_epochs(o2) == 30);
-  check(umapgpu_optimizer_step_epochs(o2, 30));
-  umapgpu_fitted* f = nullptr;
-  const umapgpu_config c = config.to_c();
-  check(umapgpu_optimizer_into_fitted(o2, &c, &f));
-  FittedUmap fitted(f);
+  // Serialize (bincode::serialize) and deserialize (bincode::deserialize)
+  std::vector<uint8_t> serialized = opt.to_bincode();
+  Optimizer opt2 = Optimizer::from_bincode(serialized);
+  REQUIRE(opt2.current_epoch() == 20);
+  REQUIRE(opt2.remaining_epochs() == 30);
+  // Continue training
+  opt2.step_epochs(30, metric);
+  FittedUmap fitted = std::move(opt2).into_fitted(config);
   REQUIRE((fitted.embedding().shape() == std::vector<size_t>{50, 2}));
 }
 
@@ -156,12 +135,8 @@ static void test_manifold_serialization() {  // tests.rs:164-188
   Umap umap(config10());
   LearnedManifold manifold = umap.learn_manifold(t.data_view(), t.idx_view(), t.dist_view());
   auto ab = manifold.curve_params();
-  SparseMat g = manifold.graph();
-  auto sr = manifold.sigmas_rhos();
-  umapgpu_manifold* h = nullptr;
-  check(umapgpu_manifold_from_host(g.n, g.nnz(), g.indptr.data(), g.indices.data(), g.data.data(), sr.first.data(),
-                                   sr.second.data(), ab.first, ab.second, &h));
-  LearnedManifold manifold2(h);
+  std::vector<uint8_t> serialized = manifold.to_bincode();
+  LearnedManifold manifold2 = LearnedManifold::from_bincode(serialized);
   REQUIRE(manifold2.curve_params() == ab);
   REQUIRE(manifold2.n_vertices() == manifold.n_vertices());
   REQUIRE(manifold2.graph().nnz() == manifold.graph().nnz());
@@ -173,19 +148,9 @@ static void test_fitted_umap_serialization() {  // tests.rs:190-215
   config.optimization.n_epochs = 10;
   Umap umap(config);
   FittedUmap fitted = umap.fit(t.data_view(), t.idx_view(), t.dist_view(), t.init.view());
-  Array2f e = fitted.embedding();
-  LearnedManifold m = fitted.manifold();
-  SparseMat g = m.graph();
-  auto sr = m.sigmas_rhos();
-  auto ab = m.curve_params();
-  umapgpu_manifold* h = nullptr;
-  check(umapgpu_manifold_from_host(g.n, g.nnz(), g.indptr.data(), g.indices.data(), g.data.data(), sr.first.data(),
-                                   sr.second.data(), ab.first, ab.second, &h));
-  umapgpu_fitted* f2 = nullptr;
-  const umapgpu_config c = fitted.config().to_c();
-  check(umapgpu_fitted_from_host(h, e.data.data(), e.rows, e.cols, &c, &f2));
-  FittedUmap fitted2(f2);
-  REQUIRE(fitted2.embedding().shape() == e.shape());
+  std::vector<uint8_t> serialized = fitted.to_bincode();
+  FittedUmap fitted2 = FittedUmap::from_bincode(serialized);
+  REQUIRE(fitted2.embedding().shape() == fitted.embedding().shape());
   REQUIRE(fitted2.manifold().n_vertices() == fitted.manifold().n_vertices());
 }
 

@@ -66,17 +66,20 @@ def test_checkpoint_serialization(gpu):  # tests.rs:132-162
     metric = gpu.EuclideanMetric()
     opt = gpu.Optimizer.new(m, init, 50, cfg, metric.metric_type())
     opt.step_epochs(20, metric)
-    state = opt.to_state()
-    opt2 = gpu.Optimizer.from_state(state)
+    serialized = opt.to_bincode()  # bincode::serialize(&opt)
+    opt2 = gpu.Optimizer.from_bincode(serialized)
     assert opt2.current_epoch() == 20 and opt2.remaining_epochs() == 30
     opt2.step_epochs(30, metric)
     assert opt2.into_fitted(cfg).embedding().shape == (50, 2)
+    opt3 = gpu.Optimizer.from_state(opt.to_state())  # the host-array form
+    assert opt3.current_epoch() == 20 and opt3.remaining_epochs() == 30
 
 
 def test_manifold_serialization(gpu):  # tests.rs:164-188
     data, idx, d, _ = ring_test_data(seed=6)
     m = gpu.Umap.new(_cfg(gpu)).learn_manifold(data, idx, d)
-    m2 = gpu.LearnedManifold.from_state(m.to_state())
+    m2 = gpu.LearnedManifold.from_bincode(m.to_bincode())
+    assert gpu.LearnedManifold.from_state(m.to_state()).graph().nnz() == m.graph().nnz()
     assert m2.curve_params() == m.curve_params()
     assert m2.n_vertices() == m.n_vertices()
     assert m2.graph().nnz() == m.graph().nnz()
@@ -86,7 +89,8 @@ def test_manifold_serialization(gpu):  # tests.rs:164-188
 def test_fitted_umap_serialization(gpu):  # tests.rs:190-215
     data, idx, d, init = ring_test_data(seed=7)
     fitted = gpu.Umap.new(_cfg(gpu, n_epochs=10)).fit(data, idx, d, init)
-    f2 = gpu.FittedUmap.from_state(fitted.to_state())
+    f2 = gpu.FittedUmap.from_bincode(fitted.to_bincode())
+    assert gpu.FittedUmap.from_state(fitted.to_state()).config() == fitted.config()
     assert f2.embedding().shape == fitted.embedding().shape
     assert f2.manifold().n_vertices() == fitted.manifold().n_vertices()
     assert f2.config() == fitted.config()

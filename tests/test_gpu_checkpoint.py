@@ -1,0 +1,199 @@
+"""Checkpoints in the reference's bytes (bincode 1.3 over the serde derives of
+Optimizer, LearnedManifold and FittedUmap; tests.rs:132-215,
+README.md:60-126), through the C ABI.
+
+* The library's bytes equal the restatement of the published layouts
+  (oracle/bincode_oracle.py, pinned by hand-assembled dumps), built from the
+  handle's own exported arrays.
+* Round trips restore every array bit for bit; an optimizer restored at
+  epoch 20 and stepped to 50 ends bit-identical to the run that was never
+  interrupted (SERIAL and the default epoch), with the same seed.
+* Malformed inputs are rejected with a "bincode: ..." message: truncation at
+  any point, array version, shape/data mismatch, bool / Option / enum tags,
+  a CSC or invalid graph.  Trailing bytes are ignored, as
+  bincode::deserialize does.
+"""
+import dataclasses
+import os
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import gmm, knn_exact, ring_test_data, uniform_init
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+import bincode_oracle as B  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(U, **opt):
+    c = U.UmapConfig()
+    c.graph.n_neighbors = 10
+    for k, v in opt.items():
+        setattr(c.optimization, k, v)
+    return c
+
+
+def _manifold_fields(m):
+    st = m.to_state()
+    return {k: st[k] for k in ("indptr", "indices", "data", "sigmas", "rhos", "n_vertices", "a", "b")}
+
+
+def _optimizer_fields(o):
+    st = o.to_state()
+    f = {k: st[k] for k in ("head", "tail", "epochs_per_sample", "embedding", "epoch_of_next_sample",
+                            "epoch_of_next_negative_sample", "epochs_per_negative_sample", "current_epoch",
+                            "total_epochs", "gamma", "initial_alpha", "negative_sample_rate")}
+    f["metric_type"] = int(st["metric_type"])
+    f["manifold"] = _manifold_fields(o.manifold())
+    return f
+
+
+def _same_state(a, b):
+    for k in ("head", "tail", "epochs_per_sample", "epoch_of_next_sample", "epoch_of_next_negative_sample",
+              "epochs_per_negative_sample", "embedding"):
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+    for k in ("current_epoch", "total_epochs", "gamma", "initial_alpha", "negative_sample_rate", "metric_type"):
+        assert a[k] == b[k], k
+    for k in ("indptr", "indices", "data", "sigmas", "rhos"):
+        assert np.array_equal(a["manifold"][k].view(np.uint8), b["manifold"][k].view(np.uint8)), k
+
+
+def test_manifold_bytes_and_round_trip(gpu):  # tests.rs:164-188
+    data, idx, d, _ = ring_test_data(seed=6)
+    m = gpu.Umap.new(_cfg(gpu)).learn_manifold(data, idx, d)
+    raw = m.to_bincode()
+    assert raw == B.manifold_bytes(_manifold_fields(m))
+    m2 = gpu.LearnedManifold.from_bincode(raw)
+    assert m2.curve_params() == m.curve_params()
+    assert m2.n_vertices() == m.n_vertices()
+    assert m2.graph().nnz() == m.graph().nnz()
+    assert m2.to_bincode() == raw
+
+
+@pytest.mark.parametrize("serial", [True, False])
+def test_optimizer_checkpoint_resumes_bit_identically(gpu, serial):  # tests.rs:132-162
+    x, _ = gmm(1500, 16, 5, seed=3)
+    idx, d = knn_exact(x, 15)
+    cfg = _cfg(gpu)
+    cfg.graph.n_neighbors = 15
+    m = gpu.Umap.new(cfg).learn_manifold(x, idx, d)
+    init = uniform_init(1500, 2, seed=9)
+    a = gpu.Optimizer.new(m.clone(), init, 50, cfg, gpu.MetricType.Euclidean, seed=11)
+    b = gpu.Optimizer.new(m, init, 50, cfg, gpu.MetricType.Euclidean, seed=11)
+    for o in (a, b):
+        if serial:
+            o.set_mode(gpu.SGD_SERIAL)
+        o.step_epochs(20)
+    raw = a.to_bincode()
+    assert raw == B.optimizer_bytes(_optimizer_fields(a))
+    a2 = gpu.Optimizer.from_bincode(raw, seed=11)
+    assert a2.current_epoch() == 20 and a2.remaining_epochs() == 30
+    _same_state(_optimizer_fields(a2), _optimizer_fields(a))
+    assert a2.to_bincode() == raw
+    if serial:
+        a2.set_mode(gpu.SGD_SERIAL)
+    a2.step_epochs(30)
+    b.step_epochs(30)
+    assert np.array_equal(a2.embedding().view(np.uint32), b.embedding().view(np.uint32))
+    assert a2.into_fitted(cfg).embedding().shape == (1500, 2)
+
+
+def test_generic_metric_type_survives(gpu):
+    data, idx, d, init = ring_test_data(seed=5)
+    cfg = _cfg(gpu)
+    m = gpu.Umap.new(cfg).learn_manifold(data, idx, d)
+    o = gpu.Optimizer.new(m, init, 10, cfg, gpu.MetricType.Generic)
+    raw = o.to_bincode()
+    assert struct.unpack("<I", raw[-4:])[0] == 1  # MetricType::Generic is the last field
+    assert gpu.Optimizer.from_bincode(raw).export_params()["metric_type"] == gpu.MetricType.Generic
+
+
+def test_fitted_bytes_and_round_trip(gpu):  # tests.rs:190-215
+    data, idx, d, init = ring_test_data(seed=7)
+    cfg = _cfg(gpu, n_epochs=10)
+    cfg.graph.disconnection_distance = 1.9  # a Some(..) among the Options
+    fitted = gpu.Umap.new(cfg).fit(data, idx, d, init)
+    raw = fitted.to_bincode()
+    expect = B.fitted_bytes({"embedding": fitted.embedding(), "manifold": _manifold_fields(fitted.manifold()),
+                             "config": dataclasses.asdict(fitted.config())})
+    assert raw == expect
+    f2 = gpu.FittedUmap.from_bincode(raw)
+    assert np.array_equal(f2.embedding(), fitted.embedding(), equal_nan=True)
+    assert f2.manifold().n_vertices() == fitted.manifold().n_vertices()
+    assert f2.config() == fitted.config()
+    assert f2.to_bincode() == raw
+
+
+def test_size_query_and_small_buffer(gpu):
+    import ctypes as C
+    data, idx, d, _ = ring_test_data(seed=6)
+    m = gpu.Umap.new(_cfg(gpu)).learn_manifold(data, idx, d)
+    n = C.c_uint64()
+    gpu._check(gpu.lib().umapgpu_manifold_to_bincode(m._handle(), None, 0, C.byref(n)))
+    assert n.value == len(m.to_bincode())
+    small = (C.c_uint8 * 8)()
+    with pytest.raises(gpu.UmapPanic, match="buffer too small"):
+        gpu._check(gpu.lib().umapgpu_manifold_to_bincode(m._handle(), small, 8, C.byref(n)))
+
+
+def test_malformed_inputs_are_rejected(gpu):
+    data, idx, d, init = ring_test_data(seed=8)
+    cfg = _cfg(gpu)
+    m = gpu.Umap.new(cfg).learn_manifold(data, idx, d)
+    o = gpu.Optimizer.new(m.clone(), init, 10, cfg, gpu.MetricType.Euclidean)
+    mb, ob = m.to_bincode(), o.to_bincode()
+    # truncation anywhere (a sample of cut points, every field boundary region included)
+    cuts = sorted(set([0, 1, 3, 4, 11, 12, 19, 20, 27, 28] + list(range(0, len(ob), max(1, len(ob) // 97)))))
+    for cut in cuts:
+        if cut >= len(ob):
+            continue
+        with pytest.raises(gpu.UmapPanic, match="bincode: unexpected end of input"):
+            gpu.Optimizer.from_bincode(ob[:cut])
+    # trailing bytes are ignored
+    assert gpu.LearnedManifold.from_bincode(mb + b"\x07" * 5).to_bincode() == mb
+    # CSC storage, then an unknown variant
+    with pytest.raises(gpu.UmapPanic, match="must be a CSR matrix"):
+        gpu.LearnedManifold.from_bincode(struct.pack("<I", 1) + mb[4:])
+    with pytest.raises(gpu.UmapPanic, match="unknown variant index 2 of CompressedStorage"):
+        gpu.LearnedManifold.from_bincode(struct.pack("<I", 2) + mb[4:])
+    # graph structure: a column index out of range
+    n = m.n_vertices()
+    nnz = m.graph().nnz()
+    idx_off = 4 + 8 + 8 + 8 + 8 * (n + 1) + 8  # first u32 of indices
+    bad = bytearray(mb)
+    bad[idx_off:idx_off + 4] = struct.pack("<I", n + 5)
+    with pytest.raises(gpu.UmapPanic, match="column index out of range"):
+        gpu.LearnedManifold.from_bincode(bytes(bad))
+    # the sigmas array: version byte, then a dimension that disagrees with the data
+    sig_off = idx_off + 4 * nnz + 8 + 4 * nnz
+    bad = bytearray(mb)
+    bad[sig_off] = 2
+    with pytest.raises(gpu.UmapPanic, match="unknown array version: 2"):
+        gpu.LearnedManifold.from_bincode(bytes(bad))
+    bad = bytearray(mb)
+    bad[sig_off + 1:sig_off + 9] = struct.pack("<Q", n + 1)
+    with pytest.raises(gpu.UmapPanic, match="data and dimension must match"):
+        gpu.LearnedManifold.from_bincode(bytes(bad))
+    # MetricType variant 2
+    with pytest.raises(gpu.UmapPanic, match="unknown variant index 2 of MetricType"):
+        gpu.Optimizer.from_bincode(ob[:-4] + struct.pack("<I", 2))
+    # FittedUmap: a bool that is neither 0 nor 1 (symmetrize), an Option tag of 2 (a)
+    fitted = gpu.Umap.new(_cfg(gpu, n_epochs=5)).fit(data, idx, d, init)
+    fb = fitted.to_bincode()
+    cfg_len = len(B.config_bytes(dataclasses.asdict(fitted.config())))
+    c0 = len(fb) - cfg_len
+    sym = c0 + 8 + 4 + 4 + 1 + 1 + 8 + 4 + 4 + 1  # after n_components, manifold params, graph params
+    bad = bytearray(fb)
+    assert bad[sym] == 1
+    bad[sym] = 3
+    with pytest.raises(gpu.UmapPanic, match="expected a boolean"):
+        gpu.FittedUmap.from_bincode(bytes(bad))
+    bad = bytearray(fb)
+    bad[c0 + 16] = 2
+    with pytest.raises(gpu.UmapPanic, match="invalid Option tag 2"):
+        gpu.FittedUmap.from_bincode(bytes(bad))

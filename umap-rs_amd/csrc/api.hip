@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "checkpoint.hpp"
 #include "../../include/umapgpu.h"
 #include "evaluate.hpp"
 #include "graph.hpp"
@@ -433,6 +434,55 @@ std::unique_ptr<umapgpu_fitted> into_fitted(std::unique_ptr<umapgpu_optimizer> o
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Checkpoint helpers: host copies of a handle's state and back.
+namespace {
+
+void ok(umapgpu_status st) {
+  if (st != UMAPGPU_OK) throw Error((Err)st, umapgpu_last_error());
+}
+
+bincode::Manifold manifold_to_host(const umapgpu_manifold& m) {
+  bincode::Manifold h;
+  const uint64_t n = m.n_vertices, nnz = m.g.graph.nnz;
+  h.nrows = h.ncols = h.n_vertices = n;
+  h.indptr.resize(n + 1);
+  h.indices.resize(nnz);
+  h.data.resize(nnz);
+  h.sigmas.resize(n);
+  h.rhos.resize(n);
+  ok(umapgpu_manifold_copy_graph(&m, h.indptr.data(), h.indices.data(), h.data.data()));
+  ok(umapgpu_manifold_copy_sigmas_rhos(&m, h.sigmas.data(), h.rhos.data()));
+  h.a = m.a;
+  h.b = m.b;
+  return h;
+}
+
+umapgpu_manifold* manifold_from_host(const bincode::Manifold& h) {
+  umapgpu_manifold* m = nullptr;
+  ok(umapgpu_manifold_from_host(h.n_vertices, h.indices.size(), h.indptr.data(), h.indices.data(), h.data.data(),
+                                h.sigmas.data(), h.rhos.data(), h.a, h.b, &m));
+  return m;
+}
+
+// Copies the encoded bytes out (or only reports their count when buf is null).
+void emit(const bincode::Writer& w, uint64_t expect, uint8_t* buf, uint64_t* len) {
+  if (w.out.size() != expect) throw Error(Err::Hip, "internal error: bincode size mismatch");
+  *len = w.out.size();
+  std::memcpy(buf, w.out.data(), w.out.size());
+}
+
+bool size_only(uint64_t need, uint8_t* buf, uint64_t cap, uint64_t* len) {
+  if (!len) throw Error(Err::InvalidArgument, "len must not be null");
+  *len = need;
+  if (!buf) return true;
+  if (cap < need)
+    throw Error(Err::InvalidArgument,
+                "buffer too small for the serialized bytes: need " + rs_u64(need) + ", have " + rs_u64(cap));
+  return false;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* umapgpu_version(void) { return "umapgpu 0.1.0 (gfx950; reference umap-rs 0.4.5)"; }
@@ -845,6 +895,113 @@ umapgpu_status umapgpu_fitted_copy_embedding(const umapgpu_fitted* f, float* out
 }
 
 const umapgpu_manifold* umapgpu_fitted_manifold(const umapgpu_fitted* f) { return f->manifold.get(); }
+
+// ---- bincode checkpoints (helpers above the extern "C" block) -------------
+
+umapgpu_status umapgpu_manifold_to_bincode(const umapgpu_manifold* m, uint8_t* buf, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    if (!m) throw Error(Err::InvalidArgument, "null handle");
+    const uint64_t need = bincode::manifold_bytes(m->n_vertices, m->g.graph.nnz);
+    if (size_only(need, buf, cap, len)) return;
+    bincode::Writer w;
+    w.out.reserve(need);
+    bincode::write_manifold(w, manifold_to_host(*m));
+    emit(w, need, buf, len);
+  });
+}
+
+umapgpu_status umapgpu_manifold_from_bincode(const uint8_t* buf, uint64_t len, umapgpu_manifold** out) {
+  return guarded([&] {
+    if (!buf && len) throw Error(Err::InvalidArgument, "null buffer");
+    bincode::Reader r(buf, len);
+    const bincode::Manifold h = bincode::read_manifold(r);
+    *out = manifold_from_host(h);
+  });
+}
+
+umapgpu_status umapgpu_optimizer_to_bincode(const umapgpu_optimizer* o, uint8_t* buf, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    if (!o) throw Error(Err::InvalidArgument, "null handle");
+    const OptimizerState& st = o->st;
+    const uint64_t need = bincode::optimizer_bytes(st.n, o->manifold->g.graph.nnz, st.n_edges, st.dim);
+    if (size_only(need, buf, cap, len)) return;
+    bincode::Optimizer h;
+    h.manifold = manifold_to_host(*o->manifold);
+    const uint64_t E = st.n_edges;
+    h.head.resize(E);
+    h.tail.resize(E);
+    h.epochs_per_sample.resize(E);
+    h.epoch_of_next_sample.resize(E);
+    h.epoch_of_next_negative_sample.resize(E);
+    h.epochs_per_negative_sample.resize(E);
+    h.rows = st.n;
+    h.cols = st.dim;
+    h.embedding.resize(st.n * st.dim);
+    ensure_device();
+    optimizer_export(st, h.head.data(), h.tail.data(), h.epochs_per_sample.data(), h.epoch_of_next_sample.data(),
+                     h.epochs_per_negative_sample.data(), h.epoch_of_next_negative_sample.data(), h.embedding.data(),
+                     stream());
+    UMAP_HIP(hipStreamSynchronize(stream()));
+    h.current_epoch = st.current_epoch;
+    h.total_epochs = st.total_epochs;
+    h.gamma = st.gamma;
+    h.initial_alpha = st.initial_alpha;
+    h.negative_sample_rate = st.negative_sample_rate;
+    h.metric_type = (uint32_t)st.metric_type;
+    bincode::Writer w;
+    w.out.reserve(need);
+    bincode::write_optimizer(w, h);
+    emit(w, need, buf, len);
+  });
+}
+
+umapgpu_status umapgpu_optimizer_from_bincode(const uint8_t* buf, uint64_t len, uint64_t seed,
+                                              umapgpu_optimizer** out) {
+  return guarded([&] {
+    if (!buf && len) throw Error(Err::InvalidArgument, "null buffer");
+    bincode::Reader r(buf, len);
+    const bincode::Optimizer h = bincode::read_optimizer(r);
+    umapgpu_manifold* m = manifold_from_host(h.manifold);
+    const umapgpu_status st = umapgpu_optimizer_import_state(
+        m, h.head.size(), h.head.data(), h.tail.data(), h.epochs_per_sample.data(), h.epoch_of_next_sample.data(),
+        h.epochs_per_negative_sample.data(), h.epoch_of_next_negative_sample.data(), h.embedding.data(), h.cols,
+        h.current_epoch, h.total_epochs, h.gamma, h.initial_alpha, h.negative_sample_rate, (int32_t)h.metric_type,
+        seed, out);
+    if (st != UMAPGPU_OK) umapgpu_manifold_destroy(m);  // not consumed on failure
+    ok(st);
+  });
+}
+
+umapgpu_status umapgpu_fitted_to_bincode(const umapgpu_fitted* f, uint8_t* buf, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    if (!f) throw Error(Err::InvalidArgument, "null handle");
+    const uint64_t need = bincode::fitted_bytes(f->n, f->manifold->g.graph.nnz, f->dim, f->config);
+    if (size_only(need, buf, cap, len)) return;
+    bincode::Fitted h;
+    h.rows = f->n;
+    h.cols = f->dim;
+    h.embedding.resize(f->n * f->dim);
+    ok(umapgpu_fitted_copy_embedding(f, h.embedding.data()));
+    h.manifold = manifold_to_host(*f->manifold);
+    h.config = f->config;
+    bincode::Writer w;
+    w.out.reserve(need);
+    bincode::write_fitted(w, h);
+    emit(w, need, buf, len);
+  });
+}
+
+umapgpu_status umapgpu_fitted_from_bincode(const uint8_t* buf, uint64_t len, umapgpu_fitted** out) {
+  return guarded([&] {
+    if (!buf && len) throw Error(Err::InvalidArgument, "null buffer");
+    bincode::Reader r(buf, len);
+    const bincode::Fitted h = bincode::read_fitted(r);
+    umapgpu_manifold* m = manifold_from_host(h.manifold);
+    const umapgpu_status st = umapgpu_fitted_from_host(m, h.embedding.data(), h.rows, h.cols, &h.config, out);
+    if (st != UMAPGPU_OK) umapgpu_manifold_destroy(m);
+    ok(st);
+  });
+}
 
 umapgpu_status umapgpu_fitted_config(const umapgpu_fitted* f, umapgpu_config* cfg) {
   return guarded([&] { *cfg = f->config; });

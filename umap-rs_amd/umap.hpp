@@ -195,6 +195,18 @@ struct SparseMat {
   std::pair<size_t, size_t> shape() const { return {n, n}; }
 };
 
+namespace detail {
+// bincode::serialize through a *_to_bincode entry point: the size, then the bytes.
+template <typename H, typename F>
+std::vector<uint8_t> to_bincode(F fn, H h) {
+  uint64_t n = 0;
+  check(fn(h, nullptr, 0, &n));
+  std::vector<uint8_t> out(n);
+  check(fn(h, out.data(), n, &n));
+  return out;
+}
+}  // namespace detail
+
 class LearnedManifold {  // manifold.rs:15-54, device resident
  public:
   LearnedManifold() = default;
@@ -250,6 +262,13 @@ class LearnedManifold {  // manifold.rs:15-54, device resident
     check(umapgpu_manifold_copy_sigmas_rhos(handle(), s.data(), r.data()));
     return {s, r};
   }
+  // bincode::serialize(&manifold) / bincode::deserialize (manifold.rs:14)
+  std::vector<uint8_t> to_bincode() const { return detail::to_bincode(umapgpu_manifold_to_bincode, handle()); }
+  static LearnedManifold from_bincode(const std::vector<uint8_t>& bytes) {
+    umapgpu_manifold* m = nullptr;
+    check(umapgpu_manifold_from_bincode(bytes.data(), bytes.size(), &m));
+    return LearnedManifold(m);
+  }
   umapgpu_manifold* handle() const {
     if (!h_) throw Panic("LearnedManifold was moved and can no longer be used");
     return h_;
@@ -297,6 +316,13 @@ class FittedUmap {  // embedding.rs:439-523
     check(umapgpu_fitted_config(h(), &c));
     return UmapConfig::from_c(c);
   }
+  // bincode::serialize(&fitted) / bincode::deserialize (embedding.rs:438)
+  std::vector<uint8_t> to_bincode() const { return detail::to_bincode(umapgpu_fitted_to_bincode, h()); }
+  static FittedUmap from_bincode(const std::vector<uint8_t>& bytes) {
+    umapgpu_fitted* f = nullptr;
+    check(umapgpu_fitted_from_bincode(bytes.data(), bytes.size(), &f));
+    return FittedUmap(f);
+  }
   // embedding.rs:516-523: not implemented in the reference either
   Array2f transform(ArrayView2<float>, ArrayView2<uint32_t>, ArrayView2<float>) const {
     throw Panic("not yet implemented: Transform not yet implemented - contributions welcome!");
@@ -325,6 +351,14 @@ class Optimizer {  // optimizer.rs:28-403
   }
   Optimizer(Optimizer&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
   Optimizer(const Optimizer&) = delete;
+  // bincode::serialize(&opt) / bincode::deserialize (optimizer.rs:27); the
+  // checkpoint has no RNG state, so `seed` keys the restored negative sampling.
+  std::vector<uint8_t> to_bincode() const { return detail::to_bincode(umapgpu_optimizer_to_bincode, h()); }
+  static Optimizer from_bincode(const std::vector<uint8_t>& bytes, uint64_t seed = 0) {
+    umapgpu_optimizer* o = nullptr;
+    check(umapgpu_optimizer_from_bincode(bytes.data(), bytes.size(), seed, &o));
+    return Optimizer(o);
+  }
   ~Optimizer() {
     if (h_) umapgpu_optimizer_destroy(h_);
   }
@@ -369,6 +403,7 @@ class Optimizer {  // optimizer.rs:28-403
   umapgpu_optimizer* handle() const { return h(); }
 
  private:
+  explicit Optimizer(umapgpu_optimizer* h) : h_(h) {}
   umapgpu_optimizer* h() const {
     if (!h_) throw Panic("Optimizer was moved and can no longer be used");
     return h_;

@@ -133,6 +133,12 @@ def lib():
             "umapgpu_fitted_config": ([P, C.POINTER(_Config)], i32),
             "umapgpu_fitted_from_host": ([P, P, u64, u64, C.POINTER(_Config), C.POINTER(P)], i32),
             "umapgpu_fitted_destroy": ([P], None),
+            "umapgpu_manifold_to_bincode": ([P, P, u64, C.POINTER(u64)], i32),
+            "umapgpu_manifold_from_bincode": ([P, u64, C.POINTER(P)], i32),
+            "umapgpu_optimizer_to_bincode": ([P, P, u64, C.POINTER(u64)], i32),
+            "umapgpu_optimizer_from_bincode": ([P, u64, u64, C.POINTER(P)], i32),
+            "umapgpu_fitted_to_bincode": ([P, P, u64, C.POINTER(u64)], i32),
+            "umapgpu_fitted_from_bincode": ([P, u64, C.POINTER(P)], i32),
             "umapgpu_device_alloc": ([u64, C.POINTER(P)], i32),
             "umapgpu_device_free": ([P], i32),
             "umapgpu_memcpy_htod": ([P, P, u64], i32),
@@ -193,6 +199,24 @@ def _check(status: int) -> None:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(P)
+
+
+def _to_bincode(fn, h) -> bytes:
+    """bincode::serialize through a *_to_bincode entry point (size, then bytes)."""
+    n = u64()
+    _check(fn(h, None, 0, C.byref(n)))
+    buf = (C.c_uint8 * max(n.value, 1))()
+    _check(fn(h, buf, n.value, C.byref(n)))
+    return bytes(memoryview(buf)[:n.value])
+
+
+def _from_bincode(fn, data, *extra):
+    """bincode::deserialize through a *_from_bincode entry point; returns the raw handle."""
+    data = bytes(data)
+    buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+    out = P()
+    _check(fn(buf, len(data), *extra, C.byref(out)))
+    return out
 
 
 def device_count() -> int:
@@ -457,6 +481,15 @@ class LearnedManifold(_Handle):
         return {"indptr": g.indptr(), "indices": g.indices(), "data": g.data(), "sigmas": s, "rhos": r,
                 "n_vertices": np.uint64(nv), "a": np.float32(a), "b": np.float32(b)}
 
+    def to_bincode(self) -> bytes:
+        """bincode::serialize(&manifold): the reference's checkpoint bytes (manifold.rs:14)."""
+        return _to_bincode(lib().umapgpu_manifold_to_bincode, self._handle())
+
+    @staticmethod
+    def from_bincode(data: bytes) -> "LearnedManifold":
+        """bincode::deserialize::<LearnedManifold>."""
+        return LearnedManifold(_from_bincode(lib().umapgpu_manifold_from_bincode, data))
+
     @staticmethod
     def from_state(st: dict) -> "LearnedManifold":
         ip = np.ascontiguousarray(st["indptr"], np.uint64)
@@ -506,6 +539,15 @@ class FittedUmap(_Handle):
         st = {"embedding": self.embedding(), "config": dataclasses.asdict(self.config())}
         st["manifold"] = self.manifold().to_state()
         return st
+
+    def to_bincode(self) -> bytes:
+        """bincode::serialize(&fitted) (embedding.rs:438)."""
+        return _to_bincode(lib().umapgpu_fitted_to_bincode, self._handle())
+
+    @staticmethod
+    def from_bincode(data: bytes) -> "FittedUmap":
+        """bincode::deserialize::<FittedUmap>."""
+        return FittedUmap(_from_bincode(lib().umapgpu_fitted_from_bincode, data))
 
     @staticmethod
     def from_state(st: dict) -> "FittedUmap":
@@ -651,6 +693,16 @@ class Optimizer(_Handle):
         st.update(self.export_params())
         st["manifold"] = self.manifold().to_state()
         return st
+
+    def to_bincode(self) -> bytes:
+        """bincode::serialize(&optimizer): the reference's checkpoint bytes (optimizer.rs:27)."""
+        return _to_bincode(lib().umapgpu_optimizer_to_bincode, self._handle())
+
+    @staticmethod
+    def from_bincode(data: bytes, seed: int = 0) -> "Optimizer":
+        """bincode::deserialize::<Optimizer>; the checkpoint carries no RNG state, so `seed`
+        keys the negative-sample generator of the restored optimizer."""
+        return Optimizer(_from_bincode(lib().umapgpu_optimizer_from_bincode, data, seed))
 
     @staticmethod
     def from_state(st: dict) -> "Optimizer":
