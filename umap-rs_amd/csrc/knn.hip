@@ -118,7 +118,9 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
 //
 // Layout: a block is 4 waves x 32 queries; candidates are staged 32 at a
 // time in LDS transposed (dimension-major, row stride 48 floats so that the
-// four k-rows a B operand read touches fall on distinct banks).  MFMA lane
+// four k-rows a B operand read touches fall on distinct banks), the next
+// tile's loads in flight while the current one is ranked.  Both halves of a
+// wave rank: lanes q and q + 32 each keep a list for one half of every tile.  MFMA lane
 // layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16] and
 // receives C[4*(l/16) + r][l%16] in register r.
 constexpr int kMQ = 128;     // queries per block (32 per wave)
@@ -126,6 +128,23 @@ constexpr int kMC = 32;      // candidates per tile
 constexpr int kMCS = 48;     // LDS row stride of the transposed candidate tile
 
 typedef float knn_v4f __attribute__((ext_vector_type(4)));
+
+// One step of a sorted (value, index) list: (v, i) enters at its place and
+// the largest entry leaves.  Compile-time positions keep the list in
+// registers.
+template <int KM>
+__device__ inline void list_insert(float (&bd)[KM], uint32_t (&bi)[KM], float v, uint32_t i) {
+#pragma unroll
+  for (int p = 0; p < KM; ++p) {
+    const bool sw = v < bd[p] || (v == bd[p] && i < bi[p]);
+    const float td = bd[p];
+    const uint32_t ti = bi[p];
+    bd[p] = sw ? v : td;
+    bi[p] = sw ? i : ti;
+    v = sw ? td : v;
+    i = sw ? ti : i;
+  }
+}
 
 template <int DIM, int KM>
 __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__ X, const float* __restrict__ norms,
@@ -135,6 +154,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
                                                        uint32_t* __restrict__ redo_count) {
   constexpr int DP = (DIM + 3) / 4 * 4;  // K padded to the MFMA step
   constexpr int KS = DP / 4;             // MFMA steps per 16x16 tile
+  constexpr int NPT = (kMC * DP + 255) / 256;  // tile values staged per thread
   __shared__ float s_ct[DP * kMCS];      // candidates, transposed
   __shared__ float s_cn[kMC];            // their squared norms
   __shared__ float s_qn[kMQ];            // the block's query norms
@@ -156,9 +176,14 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     const uint32_t qr = blockIdx.x * kMQ + threadIdx.x;
     s_qn[threadIdx.x] = qr < n ? norms[qr] : 0.0f;
   }
-  // per query (lanes 0..31 of each wave): the KM smallest approximations
-  const uint32_t myq = qbase + lane;
-  const bool owner = lane < 32 && myq < n;
+  // Query ql of the wave is served by lanes ql and ql + 32: each scans one
+  // half of every tile and keeps the KM smallest approximations of its half.
+  // The KM smallest of the union are within the two lists, which are merged
+  // at the end.
+  const int ql = lane & 31;
+  const uint32_t half = (uint32_t)lane >> 5;
+  const uint32_t myq = qbase + ql;
+  const bool active = myq < n;
   float bd[KM];
   uint32_t bi[KM];
 #pragma unroll
@@ -167,14 +192,31 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     bi[i] = 0xffffffffu;
   }
   float worst = INFINITY;
-  for (uint32_t c0 = 0; c0 < n; c0 += kMC) {
-    __syncthreads();  // the previous tile is done with s_ct (and s_qn is written)
-    for (uint32_t e = threadIdx.x; e < (uint32_t)(kMC * DP); e += 256) {
+  // the next tile, loaded into registers while the current one is ranked
+  float pf[NPT];
+  float pfn = 0.0f;
+  auto load_tile = [&](uint32_t c0) {
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+      const uint32_t e = threadIdx.x + 256u * (uint32_t)i;
       const uint32_t c = e / DP, d = e - c * DP;
       const uint32_t ci = c0 + c;
-      s_ct[d * kMCS + c] = (ci < n && d < (uint32_t)DIM) ? X[(uint64_t)ci * DIM + d] : 0.0f;
+      pf[i] = (e < (uint32_t)(kMC * DP) && ci < n && d < (uint32_t)DIM) ? X[(uint64_t)ci * DIM + d] : 0.0f;
     }
-    if (threadIdx.x < kMC) s_cn[threadIdx.x] = (c0 + threadIdx.x < n) ? norms[c0 + threadIdx.x] : 0.0f;
+    pfn = (threadIdx.x < (uint32_t)kMC && c0 + threadIdx.x < n) ? norms[c0 + threadIdx.x] : 0.0f;
+  };
+  load_tile(0);
+  for (uint32_t c0 = 0; c0 < n; c0 += kMC) {
+    __syncthreads();  // every wave is done with the previous tile's s_ct (and s_qn is written)
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+      const uint32_t e = threadIdx.x + 256u * (uint32_t)i;
+      if (e < (uint32_t)(kMC * DP)) {
+        const uint32_t c = e / DP, d = e - c * DP;
+        s_ct[d * kMCS + c] = pf[i];
+      }
+    }
+    if (threadIdx.x < kMC) s_cn[threadIdx.x] = pfn;
     __syncthreads();
     knn_v4f acc[2][2];
 #pragma unroll
@@ -200,32 +242,36 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
           const int row = t * 16 + 4 * (lane >> 4) + r, col = u * 16 + (lane & 15);
           s_d[w][row][col] = (s_qn[w * 32 + row] + s_cn[col]) - 2.0f * acc[t][u][r];
         }
+    if (c0 + kMC < n) load_tile(c0 + kMC);  // in flight during the ranking below
     __syncthreads();  // s_d is ready
-    if (owner) {
+    if (active) {
+      // which of this lane's 16 candidates beat its current list
+      const uint32_t lo = half * 16;
       const uint32_t nc = (n - c0) < (uint32_t)kMC ? (n - c0) : (uint32_t)kMC;
-#pragma unroll 8
-      for (uint32_t c = 0; c < nc; ++c) {
-        const float v = s_d[w][lane][c];
-        const uint32_t ci = c0 + c;
-        if (v < worst && ci != myq) {
-          float nd = v;
-          uint32_t ni = ci;
+      uint32_t m = 0;
 #pragma unroll
-          for (int i = 0; i < KM; ++i) {
-            const bool sw = nd < bd[i] || (nd == bd[i] && ni < bi[i]);
-            const float td = bd[i];
-            const uint32_t ti = bi[i];
-            bd[i] = sw ? nd : td;
-            bi[i] = sw ? ni : ti;
-            nd = sw ? td : nd;
-            ni = sw ? ti : ni;
-          }
+      for (uint32_t c = 0; c < 16; ++c)
+        m |= (lo + c < nc && s_d[w][ql][lo + c] < worst) ? (1u << c) : 0u;
+      if (myq >= c0 + lo && myq < c0 + lo + 16) m &= ~(1u << (myq - c0 - lo));  // not itself
+      while (m) {
+        const uint32_t b = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        const float v = s_d[w][ql][lo + b];
+        if (v < worst) {
+          list_insert<KM>(bd, bi, v, c0 + lo + b);
           worst = bd[KM - 1];
         }
       }
     }
   }
-  if (!owner) return;
+  // lane ql merges the list of lane ql + 32
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    const float od = __shfl_down(bd[i], 32, 64);
+    const uint32_t oi = (uint32_t)__shfl_down((int)bi[i], 32, 64);
+    if (half == 0 && (od < bd[KM - 1] || (od == bd[KM - 1] && oi < bi[KM - 1]))) list_insert<KM>(bd, bi, od, oi);
+  }
+  if (half != 0 || !active) return;
   // re-rank the listed candidates exactly (knn_kernel's sum and order)
   const float approx_max = bd[KM - 1];
   const bool listed_all = bi[KM - 1] == 0xffffffffu;  // fewer than KM other points exist
@@ -234,6 +280,8 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     float acc = INFINITY;
     if (bi[i] != 0xffffffffu) {
       acc = 0.0f;
+      // partial unroll: a full one would keep the whole query row live
+#pragma unroll 8
       for (int d = 0; d < DIM; ++d) {
         const float df = X[(uint64_t)myq * DIM + d] - X[(uint64_t)bi[i] * DIM + d];
         acc = acc + df * df;
@@ -256,7 +304,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     }
   // exactness check: every unlisted point lies at >= approx_max - delta
   const uint32_t kk = k - 1;
-  const float qn = s_qn[w * 32 + lane];
+  const float qn = s_qn[w * 32 + ql];
   const float delta = delta_scale * (qn + cmax2 + 2.0f * sqrtf(qn * cmax2));
   float kth = 0.0f;
 #pragma unroll
