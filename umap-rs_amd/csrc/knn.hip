@@ -127,6 +127,15 @@ constexpr int kMQ = 128;     // queries per block (32 per wave)
 constexpr int kMC = 32;      // candidates per tile
 constexpr int kMCS = 48;     // LDS row stride of the transposed candidate tile
 
+// Column swizzle of the transposed tile: candidate c of dimensions
+// 4j .. 4j+3 is stored in column c ^ knn_swizzle(j).  A staging store
+// instruction writes the 16 float4s of one candidate row (j = 0..15) for two
+// candidates per half-wave; without the swizzle those 32 lanes would hit two
+// or four banks, with it they hit distinct ones (even / odd columns for the
+// two candidates).  The B-operand reads of one MFMA step share one j, so they
+// see a permutation of 16 columns and stay conflict-free.
+__device__ inline uint32_t knn_swizzle(uint32_t j) { return (2u * j) & 31u; }
+
 typedef float knn_v4f __attribute__((ext_vector_type(4)));
 
 // One step of a sorted (value, index) list: (v, i) enters at its place and
@@ -154,7 +163,9 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
                                                        uint32_t* __restrict__ redo_count) {
   constexpr int DP = (DIM + 3) / 4 * 4;  // K padded to the MFMA step
   constexpr int KS = DP / 4;             // MFMA steps per 16x16 tile
-  constexpr int NPT = (kMC * DP + 255) / 256;  // tile values staged per thread
+  constexpr int DQ = DP / 4;                    // float4s per candidate row
+  constexpr int NPT = (kMC * DQ + 255) / 256;   // float4s of a tile staged per thread
+  static_assert(DIM % 4 == 0, "the matrix-core producer reads rows as float4");
   __shared__ float s_ct[DP * kMCS];      // candidates, transposed
   __shared__ float s_cn[kMC];            // their squared norms
   __shared__ float s_qn[kMQ];            // the block's query norms
@@ -192,16 +203,19 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     bi[i] = 0xffffffffu;
   }
   float worst = INFINITY;
-  // the next tile, loaded into registers while the current one is ranked
-  float pf[NPT];
+  // the next tile, loaded into registers (float4 f of a tile = candidate
+  // f / DQ, dimensions 4 (f % DQ) ..) while the current one is ranked
+  knn_v4f pf[NPT];
   float pfn = 0.0f;
   auto load_tile = [&](uint32_t c0) {
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
-      const uint32_t e = threadIdx.x + 256u * (uint32_t)i;
-      const uint32_t c = e / DP, d = e - c * DP;
+      const uint32_t f = threadIdx.x + 256u * (uint32_t)i;
+      const uint32_t c = f / DQ, j = f - c * DQ;
       const uint32_t ci = c0 + c;
-      pf[i] = (e < (uint32_t)(kMC * DP) && ci < n && d < (uint32_t)DIM) ? X[(uint64_t)ci * DIM + d] : 0.0f;
+      pf[i] = (f < (uint32_t)(kMC * DQ) && ci < n)
+                  ? *reinterpret_cast<const knn_v4f*>(X + (uint64_t)ci * DIM + 4 * j)
+                  : knn_v4f{0.0f, 0.0f, 0.0f, 0.0f};
     }
     pfn = (threadIdx.x < (uint32_t)kMC && c0 + threadIdx.x < n) ? norms[c0 + threadIdx.x] : 0.0f;
   };
@@ -210,10 +224,12 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     __syncthreads();  // every wave is done with the previous tile's s_ct (and s_qn is written)
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
-      const uint32_t e = threadIdx.x + 256u * (uint32_t)i;
-      if (e < (uint32_t)(kMC * DP)) {
-        const uint32_t c = e / DP, d = e - c * DP;
-        s_ct[d * kMCS + c] = pf[i];
+      const uint32_t f = threadIdx.x + 256u * (uint32_t)i;
+      if (f < (uint32_t)(kMC * DQ)) {
+        const uint32_t c = f / DQ, j = f - c * DQ;
+        const uint32_t col = c ^ knn_swizzle(j);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_ct[(4 * j + q) * kMCS + col] = pf[i][q];
       }
     }
     if (threadIdx.x < kMC) s_cn[threadIdx.x] = pfn;
@@ -226,8 +242,9 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       const int kr = 4 * s + (lane >> 4);
-      const float b0 = s_ct[kr * kMCS + (lane & 15)];
-      const float b1 = s_ct[kr * kMCS + 16 + (lane & 15)];
+      const uint32_t sw = knn_swizzle((uint32_t)s);
+      const float b0 = s_ct[kr * kMCS + ((uint32_t)(lane & 15) ^ sw)];
+      const float b1 = s_ct[kr * kMCS + ((16u + (uint32_t)(lane & 15)) ^ sw)];
       acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][s], b0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][s], b1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], b0, acc[1][0], 0, 0, 0);
