@@ -638,6 +638,20 @@ def main() -> int:
                            "hogwild_rows": "sgd_rows_kernel<2,u16,atomic tail>"}[args.mode],
                 "bytes_per_update": bpu, "avg_launch_us": round(1e6 * kernel_s / (epochs * len(kernel_ms)), 2),
                 "updates_per_launch": round(float(sum(updates)) / (epochs * len(kernel_ms)), 1)}
+    if args.mode == "hogwild":
+        # The epoch's random row reads at the rate a plain gather kernel reaches on tables
+        # of the same size: negatives read 8-B tail rows (n x 8 B), the twin / tail pair
+        # reads 16-B {H, T} rows (n x 16 B).  Their sum bounds the epoch from below.
+        r8, r16 = U.debug_gather_rate(n, 2), U.debug_gather_rate(n, 4)
+        neg_e, pos_e = float(sum(negs)) / (epochs * len(negs)), float(sum(updates)) / (epochs * len(updates))
+        bound_us = 1e6 * (neg_e / r8 + pos_e / r16)
+        roofline["gather_bound"] = {"us_per_epoch": round(bound_us, 1), "frac": round(bound_us / roofline["avg_launch_us"], 3),
+                                    "rate_8B_rows": round(r8, 1), "rate_16B_rows": round(r16, 1),
+                                    "note": "random-row reads only (negatives at the 8-B rate, tail pairs at the "
+                                            "16-B rate, measured by a gather kernel on tables of the same size); "
+                                            "the per-edge schedule streams come on top"}
+        log(f"gather bound: {bound_us:.1f} us per epoch ({r8 / 1e9:.1f} G/s 8-B rows, {r16 / 1e9:.1f} G/s 16-B rows), "
+            f"epoch {roofline['avg_launch_us']:.1f} us")
     traffic_file = os.path.join(REPO, "profiles", "sgd_traffic_c2.json")
     if os.path.exists(traffic_file) and args.mode == "hogwild" and (n, k, epochs) == (1_000_000, 15, 500):
         with open(traffic_file) as f:

@@ -479,6 +479,12 @@ UMAPGPU_API umapgpu_status umapgpu_memcpy_dtod(void* dst_device, const void* src
 /* Calibration hook: a plain streaming copy of `bytes` (a multiple of 16),
  * used to scale hardware traffic counters into bytes. */
 UMAPGPU_API umapgpu_status umapgpu_debug_copy_device(const void* src_device, void* dst_device, uint64_t bytes);
+/* Calibration hook: random-row gathers per second from a table of rows x
+ * row_floats (2 or 4) floats: 10.24M lanes each read 6 rows at
+ * pseudo-random indices, `reps` timed launches.  The reference rate for the
+ * SGD epoch's negative-sample (8-byte) and twin (16-byte) row reads. */
+UMAPGPU_API umapgpu_status umapgpu_debug_gather_rate(uint64_t rows, uint32_t row_floats, uint32_t reps,
+                                                     double* gathers_per_s);
 /* Test hook: y[i] = expf(x[i]) through the device's glibc-equivalent expf. */
 UMAPGPU_API umapgpu_status umapgpu_debug_expf_device(const float* x_device, float* y_device, uint64_t n);
 /* Test hook: y[i] = powf(x[i], e) through the device's glibc-equivalent powf. */

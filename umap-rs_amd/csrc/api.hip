@@ -1369,6 +1369,13 @@ umapgpu_status umapgpu_debug_copy_device(const void* src, void* dst, uint64_t by
   });
 }
 
+umapgpu_status umapgpu_debug_gather_rate(uint64_t rows, uint32_t row_floats, uint32_t reps, double* gathers_per_s) {
+  return guarded([&] {
+    ensure_device();
+    *gathers_per_s = device_gather_rate(rows, row_floats, reps, stream());
+  });
+}
+
 umapgpu_status umapgpu_debug_powf_device(const float* x, float e, float* y, uint64_t n) {
   return guarded([&] {
     ensure_device();

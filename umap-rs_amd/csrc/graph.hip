@@ -414,6 +414,28 @@ __global__ void copy_calibration_kernel(const uint4* __restrict__ src, uint4* __
     dst[i] = src[i];
 }
 
+// Random-row gathers: lane t reads G rows (R floats each) at pseudo-random
+// indices (SplitMix64 output of (seed, t, g)) mod rows, and writes their sum.
+// The reference point for the SGD epoch's negative-sample and twin reads.
+template <int R, int G>
+__global__ __launch_bounds__(256) void gather_calibration_kernel(const float* __restrict__ t, uint64_t rows,
+                                                                 float* __restrict__ out, uint64_t seed) {
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float v[G][R];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint64_t r = sm_mix(seed * kSmGamma2 + (lane * 64 + (uint64_t)g + 1) * kSmGamma) % rows;
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[g][q] = t[r * R + q];
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc = acc + v[g][q];
+  out[lane] = acc;
+}
+
 __global__ void expf_kernel(const float* __restrict__ x, float* __restrict__ y, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     y[i] = libm_expf(x[i]);
@@ -616,6 +638,36 @@ void device_copy_calibration(const void* src, void* dst, uint64_t bytes, hipStre
       static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
   UMAP_LAUNCH_CHECK();
   UMAP_HIP(hipStreamSynchronize(s));
+}
+
+double device_gather_rate(uint64_t rows, uint32_t row_floats, uint32_t reps, hipStream_t s) {
+  constexpr int kG = 6;
+  constexpr uint64_t kLanes = 256ull * 40000;
+  if (rows == 0 || (row_floats != 2 && row_floats != 4) || reps == 0)
+    throw Error(Err::InvalidArgument, "gather calibration needs rows > 0, row_floats 2 or 4, reps > 0");
+  DevBuf table(rows * row_floats * 4), out(kLanes * 4);
+  UMAP_HIP(hipMemsetAsync(table.p, 0, table.bytes, s));
+  const unsigned grid = (unsigned)(kLanes / 256);
+  auto launch = [&](uint64_t seed) {
+    if (row_floats == 2)
+      gather_calibration_kernel<2, kG><<<grid, 256, 0, s>>>(table.as<float>(), rows, out.as<float>(), seed);
+    else
+      gather_calibration_kernel<4, kG><<<grid, 256, 0, s>>>(table.as<float>(), rows, out.as<float>(), seed);
+    UMAP_LAUNCH_CHECK();
+  };
+  launch(1);  // warm
+  hipEvent_t a, b;
+  UMAP_HIP(hipEventCreate(&a));
+  UMAP_HIP(hipEventCreate(&b));
+  UMAP_HIP(hipEventRecord(a, s));
+  for (uint32_t i = 0; i < reps; ++i) launch(2 + i);
+  UMAP_HIP(hipEventRecord(b, s));
+  UMAP_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  UMAP_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (double)kLanes * kG * reps / (ms / 1e3);
 }
 
 void device_powf(const float* x, float e, float* y, uint64_t n, hipStream_t s) {

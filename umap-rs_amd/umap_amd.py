@@ -178,6 +178,7 @@ def lib():
             "umapgpu_sweep_destroy": ([P], None),
             "umapgpu_debug_expf_device": ([P, P, u64], i32),
             "umapgpu_debug_copy_device": ([P, P, u64], i32),
+            "umapgpu_debug_gather_rate": ([u64, u32, u32, C.POINTER(f64)], i32),
             "umapgpu_debug_powf_device": ([P, f32, P, u64], i32),
         }
         for name, (args, res) in sig.items():
@@ -1038,6 +1039,13 @@ def learn_manifold_device(config: UmapConfig, knn_idx: DeviceArray, knn_dst: Dev
     _check(lib().umapgpu_learn_manifold_device(C.byref(c), n, P(knn_idx.ptr), P(knn_dst.ptr), k, float(thr),
                                                C.byref(out)))
     return LearnedManifold(out)
+
+
+def debug_gather_rate(rows: int, row_floats: int, reps: int = 10) -> float:
+    """Random-row gathers per second from a rows x row_floats float table (calibration)."""
+    r = f64()
+    _check(lib().umapgpu_debug_gather_rate(rows, row_floats, reps, C.byref(r)))
+    return r.value
 
 
 def debug_copy(src: DeviceArray, dst: DeviceArray, nbytes: int) -> None:
