@@ -152,3 +152,20 @@ def test_builtin_rejects_unknown_metric_and_wide_parallel(gpu, oracle):
         gpu._check(gpu.lib().umapgpu_optimizer_step_epochs_builtin(opt._handle(), 1, gpu.BUILTIN_METRIC_EUCLIDEAN))
     with pytest.raises(gpu.UmapPanic, match="Cannot step 11 epochs"):
         gpu._check(gpu.lib().umapgpu_optimizer_step_epochs_builtin(opt._handle(), 11, gpu.BUILTIN_METRIC_EUCLIDEAN))
+
+
+def test_device_serial_equals_host_callback_loop(gpu, oracle):
+    """The device SERIAL generic loop and the host loop that calls a user
+    metric back (here a subclass doing EuclideanMetric's float32 arithmetic)
+    run the same sequential order with the same sqrt, divisions and glibc
+    powf, so they agree bit for bit."""
+    class UserEuclidean(gpu.EuclideanMetric):
+        pass
+
+    dev, _ = _setup(gpu, oracle, n=400, k=10, total=8, seed=5)
+    host, _ = _setup(gpu, oracle, n=400, k=10, total=8, seed=5)
+    dev.set_mode(gpu.SGD_SERIAL)
+    dev.step_epochs(4, gpu.EuclideanMetric())
+    host.step_epochs(4, UserEuclidean())
+    assert np.array_equal(dev.embedding().view(np.uint32), host.embedding().view(np.uint32))
+    assert dev.stats()[:2] == host.stats()[:2]
