@@ -78,6 +78,15 @@ __device__ inline NextT next_epoch(double eons) {
   return (NextT)ceil(eons);
 }
 
+// make_epochs_per_sample (umap/make_epochs_per_sample.rs:21-52) for one
+// weight, then epochs_per_negative_sample = eps / negative_sample_rate
+// (optimizer.rs:249-255): the operations schedule_init_kernel performs.
+__device__ inline double2 rate_from_weight(float w, double n_epochs, double max_w, double neg_rate) {
+  const double ns = n_epochs * ((double)w / max_w);
+  const double eps = (ns > 0.0) ? n_epochs / ns : -1.0;
+  return make_double2(eps, eps / neg_rate);
+}
+
 // make_epochs_per_sample + the three schedule arrays, fused.
 template <typename NextT>
 __global__ void schedule_init_kernel(const float* __restrict__ w, uint64_t e_count, double n_epochs, double max_w,
@@ -85,12 +94,10 @@ __global__ void schedule_init_kernel(const float* __restrict__ w, uint64_t e_cou
                                      NextT* __restrict__ next) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= e_count) return;
-  const double ns = n_epochs * ((double)w[i] / max_w);
-  const double eps = (ns > 0.0) ? n_epochs / ns : -1.0;
-  const double epns = eps / neg_rate;
-  rate[i] = make_double2(eps, epns);
-  sched[i] = make_double2(eps, epns);
-  next[i] = next_epoch<NextT>(eps);
+  const double2 r = rate_from_weight(w[i], n_epochs, max_w, neg_rate);
+  rate[i] = r;
+  sched[i] = r;
+  next[i] = next_epoch<NextT>(r.x);
 }
 
 template <typename NextT>
@@ -138,7 +145,12 @@ struct SgdArgs {
   double epoch_f64;
   uint64_t epoch_key;
   uint32_t ref_den;  // 1: the reference's force denominators; 0: umap-learn's
+  // Non-null: rate[e] = rate_from_weight(weight[e]) (make_epochs_per_sample,
+  // then / negative_sample_rate), recomputed instead of read.
+  const float* weight;
+  double n_epochs_f64, max_w, neg_rate;
 };
+
 
 // Per-lane view of the parameters an edge's update needs.
 struct LaneFit {
@@ -973,8 +985,12 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
           double2 sc, rt;
           sc.x = __builtin_nontemporal_load(&sched[e].x);
           sc.y = __builtin_nontemporal_load(&sched[e].y);
-          rt.x = __builtin_nontemporal_load(&rate[e].x);
-          rt.y = __builtin_nontemporal_load(&rate[e].y);
+          if (!SWEEP && A.weight) {  // block-uniform
+            rt = rate_from_weight(__builtin_nontemporal_load(A.weight + e), A.n_epochs_f64, A.max_w, A.neg_rate);
+          } else {
+            rt.x = __builtin_nontemporal_load(&rate[e].x);
+            rt.y = __builtin_nontemporal_load(&rate[e].y);
+          }
           sc.x = sc.x + rt.x;                                          // optimize_layout_euclidean.rs:308
           const uint32_t nn = sat_count((A.epoch_f64 - sc.y) / rt.y);  // :310-311
           sc.y = sc.y + (double)nn * rt.y;                             // :343
@@ -1597,6 +1613,7 @@ void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t di
   // make_epochs_per_sample + schedule arrays
   float max_w = reduce_max_f32(w.as<float>(), E, 1, -INFINITY, s);
   if (max_w <= 0.0f) max_w = 1.0f;
+  st->max_w = (double)max_w;
   st->rate = DevBuf(std::max<uint64_t>(E, 1) * 16);
   st->sched = DevBuf(std::max<uint64_t>(E, 1) * 16);
   alloc_next(st);
@@ -1611,6 +1628,8 @@ void optimizer_prepare(const DeviceCsr& g, const float* init_device, uint64_t di
           st->rate.as<double2>(), st->sched.as<double2>(), st->next.as<uint32_t>());
     UMAP_LAUNCH_CHECK();
   }
+
+  st->weight = std::move(w);  // the default epoch recomputes rates from it
 
   // embedding normalisation to [0, 10] per dimension (:184-230)
   st->head_emb = DevBuf(std::max<uint64_t>(n * dim, 1) * 4);
@@ -1711,13 +1730,17 @@ void optimizer_step(OptimizerState* st, uint64_t n_epochs, hipStream_t s, bool g
   UMAP_HIP(hipEventCreate(&ev1));
   UMAP_HIP(hipEventRecord(ev0, s));
   if (st->n && st->n_edges) {
-    SgdArgs A;
+    SgdArgs A{};
     A.a = st->a;
     A.b = st->b;
     A.m2ab = (-2.0f * st->a) * st->b;
     A.g2b = (2.0f * st->gamma) * st->b;
     A.n = (uint32_t)st->n;
     A.ref_den = (st->semantics & kSemanticsUmapLearnForces) ? 0u : 1u;
+    A.weight = st->weight.p ? st->weight.as<float>() : nullptr;
+    A.n_epochs_f64 = (double)st->total_epochs;
+    A.max_w = st->max_w;
+    A.neg_rate = st->negative_sample_rate;
     for (uint64_t e = start; e < end; ++e) {
       A.alpha = st->initial_alpha * (1.0f - ((float)e / (float)st->total_epochs));
       A.epoch = e;

@@ -60,6 +60,12 @@ struct OptimizerState {
   DevBuf row_ptr;   // u64[n+1]
   DevBuf tail;      // u32[E]
   DevBuf rate;      // double2[E]: {epochs_per_sample, epochs_per_negative_sample}
+  // f32[E]: the filtered weights make_epochs_per_sample turned into `rate`
+  // (Optimizer::new only; an imported state has none).  The default epoch
+  // recomputes an active edge's rate from its weight with the same f64
+  // operations, reading 4 bytes instead of 16.
+  DevBuf weight;
+  double max_w = 1.0;
   DevBuf sched;     // double2[E]: {epoch_of_next_sample, epoch_of_next_negative_sample}
   DevBuf next;      // u16/u32[E]: ceil(epoch_of_next_sample), the activity test
   DevBuf perm;      // u32[n]: rows by descending degree (row-parallel mode)
