@@ -57,6 +57,7 @@ def lib():
         L.orc_sgd_epoch_serial.argtypes = epoch_args + [P]
         L.orc_sgd_epoch_hogwild.argtypes = epoch_args + [P]
         L.orc_sgd_epoch_generic_euclid.argtypes = epoch_args
+        L.orc_schedule_replay.argtypes = [P, P, P, P, u64, u64, u64, P]
         L.orc_set_force_denominator.argtypes = [i32]
         L.orc_step_epochs.argtypes = [P, u64, u64, P, P, u64, P, P, P, P, f32, f32, f32, f32, u64, u64, u64, u64, i32, P,
                                       i32]
@@ -248,6 +249,18 @@ class OptimizerState:
 
 
 _STEP_MODES = {"serial": 0, "hogwild": 1, "sync": 2, "generic_sync": 3, "generic_serial": 4}
+
+
+def schedule_replay(eps, epns, eons, eonns, start_epoch: int, n_epochs: int):
+    """The schedule recurrence alone over epochs [start, start + n) (orc_schedule_replay):
+    returns (eons, eonns, positive updates, negative samples); the inputs are not modified."""
+    eps = np.ascontiguousarray(eps, np.float64)
+    epns = np.ascontiguousarray(epns, np.float64)
+    a = np.array(eons, np.float64, copy=True)
+    c = np.array(eonns, np.float64, copy=True)
+    counts = np.zeros(2, np.uint64)
+    lib().orc_schedule_replay(_ptr(eps), _ptr(epns), _ptr(a), _ptr(c), eps.size, start_epoch, n_epochs, _ptr(counts))
+    return a, c, int(counts[0]), int(counts[1])
 
 
 def rng_index(seed: int, epoch: int, edge: int, p: int, n: int) -> int:

@@ -773,6 +773,33 @@ ORC_API void orc_sgd_epoch_sync(float *head_emb, float *tail_emb, int shared, ui
                  gamma, alpha, epoch, seed, counts);
 }
 
+/* The schedule recurrence of every epoch alone (optimize_layout_euclidean.rs:
+ * 277 activity test, :308 eons += eps, :310-311 n_neg = (epoch - eonns) /
+ * epns as usize, :343 eonns += n_neg * epns), epochs [start, start + n) for
+ * every edge.  It draws no random numbers and moves no embedding, so it gives
+ * the exact positive and negative counts and the final schedule of any mode.
+ * counts[0] += positive updates, counts[1] += negative samples. */
+ORC_API void orc_schedule_replay(const double *eps, const double *epns, double *eons, double *eonns,
+                                 uint64_t n_edges, uint64_t start_epoch, uint64_t n_epochs, uint64_t *counts) {
+  uint64_t np = 0, nn = 0;
+#pragma omp parallel for schedule(static) reduction(+ : np, nn)
+  for (uint64_t i = 0; i < n_edges; i++) {
+    double a = eons[i], c = eonns[i];
+    for (uint64_t e = start_epoch; e < start_epoch + n_epochs; e++) {
+      if (!(a <= (double)e)) continue;
+      a = a + eps[i];
+      const uint64_t k = sat_u64(((double)e - c) / epns[i]);
+      c = c + (double)k * epns[i];
+      np += 1;
+      nn += k;
+    }
+    eons[i] = a;
+    eonns[i] = c;
+  }
+  counts[0] += np;
+  counts[1] += nn;
+}
+
 /* Optimizer::step_epochs (optimizer.rs:297-366): tail snapshot per call,
  * alpha = lr * (1 - epoch/total) in f32.  mode 0 = serial, 1 = Hogwild, 2 = the
  * build's synchronous row-owner epoch (orc_sgd_epoch_sync), 3 = that epoch with

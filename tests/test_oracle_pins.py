@@ -639,3 +639,23 @@ def test_generic_sync_reaches_the_serial_quality(oracle):
             ce[mode] = E.fuzzy_cross_entropy(m.indptr, m.indices, m.data, st.embedding, m.a, m.b)
         print(seed, ce)
         assert abs(ce["generic_sync"] - ce["generic_serial"]) / ce["generic_serial"] < 0.03, (seed, ce)
+
+
+def test_schedule_replay_equals_the_epochs_it_summarises(oracle):
+    """orc_schedule_replay runs only the f64 schedule recurrence; on a real
+    graph its final eons / eonns and counts equal those of the full serial
+    and Hogwild epochs (which also move the embedding), bit for bit, from the
+    state after a first call."""
+    x, _ = gmm(1500, 16, 5, seed=8)
+    idx, d = knn_exact(x, 15)
+    m = oracle.learn_manifold(idx, d)
+    init = np.random.default_rng(8).uniform(-10, 10, (1500, 2)).astype(F)
+    for mode in ("serial", "hogwild", "sync"):
+        st = oracle.OptimizerState(m, init, 60, seed=8)
+        st.step_epochs(7, mode)
+        eons0, eonns0, c0 = st.eons.copy(), st.eonns.copy(), st.counts.copy()
+        st.step_epochs(53, mode)
+        a, c, p, q = oracle.schedule_replay(st.eps, st.epns, eons0, eonns0, 7, 53)
+        assert np.array_equal(a.view(np.uint64), st.eons.view(np.uint64)), mode
+        assert np.array_equal(c.view(np.uint64), st.eonns.view(np.uint64)), mode
+        assert (p, q) == (int(st.counts[0] - c0[0]), int(st.counts[1] - c0[1])), mode
