@@ -126,6 +126,7 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
 constexpr int kMQ = 128;     // queries per block (32 per wave)
 constexpr int kMC = 32;      // candidates per tile
 constexpr int kMCS = 48;     // LDS row stride of the transposed candidate tile
+constexpr uint32_t kThrEvery = 8;  // tiles between refreshes of the cached admission bounds
 
 // Column swizzle of the transposed tile: candidate c of dimensions
 // 4j .. 4j+3 is stored in column c ^ knn_swizzle(j).  A staging store
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
   __shared__ float s_cn[kMC];            // their squared norms
   __shared__ float s_qn[kMQ];            // the block's query norms
   __shared__ float s_d[4][32][kMC + 1];  // per wave: 32 queries x 32 approximate distances
+  __shared__ float s_w[4][2][32];        // per wave: each list's largest entry (the admission bound)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t qbase = blockIdx.x * kMQ + w * 32;
   // A operands: rows qbase + t*16 + lane%16, dimension 4*s + lane/16
@@ -195,6 +197,18 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
   const uint32_t half = (uint32_t)lane >> 5;
   const uint32_t myq = qbase + ql;
   const bool active = myq < n;
+  s_w[w][half][ql] = INFINITY;
+  // The approximations this lane holds after the products are rows
+  // t*16 + 4*(lane/16) + r; thr caches, per row, the larger of its two lists'
+  // bounds.  A bound only falls, so a cached value is never below the live
+  // one: a tile whose approximations all reach their row's cached bound
+  // cannot enter a list, and the wave skips its ranking.
+  float thr[2][4];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) thr[t][r] = INFINITY;
+  uint32_t tile = 0;
   float bd[KM];
   uint32_t bi[KM];
 #pragma unroll
@@ -220,8 +234,8 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
     pfn = (threadIdx.x < (uint32_t)kMC && c0 + threadIdx.x < n) ? norms[c0 + threadIdx.x] : 0.0f;
   };
   load_tile(0);
-  for (uint32_t c0 = 0; c0 < n; c0 += kMC) {
-    __syncthreads();  // every wave is done with the previous tile's s_ct (and s_qn is written)
+  for (uint32_t c0 = 0; c0 < n; c0 += kMC, ++tile) {
+    __syncthreads();  // every wave is done with the previous tile's s_ct (s_qn, s_w are written)
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
       const uint32_t f = threadIdx.x + 256u * (uint32_t)i;
@@ -250,6 +264,17 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
       acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], b0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], b1, acc[1][1], 0, 0, 0);
     }
+    if ((tile & (kThrEvery - 1)) == 0) {  // refresh the cached bounds
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = t * 16 + 4 * (lane >> 4) + r;
+          thr[t][r] = fmaxf(s_w[w][0][row], s_w[w][1][row]);
+        }
+    }
+    float dv[2][2][4];
+    bool pass = false;
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -257,10 +282,22 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = t * 16 + 4 * (lane >> 4) + r, col = u * 16 + (lane & 15);
-          s_d[w][row][col] = (s_qn[w * 32 + row] + s_cn[col]) - 2.0f * acc[t][u][r];
+          dv[t][u][r] = (s_qn[w * 32 + row] + s_cn[col]) - 2.0f * acc[t][u][r];
+          pass = pass || (dv[t][u][r] < thr[t][r]);
         }
     if (c0 + kMC < n) load_tile(c0 + kMC);  // in flight during the ranking below
-    __syncthreads();  // s_d is ready
+    if (!__any(pass)) continue;  // wave-uniform: nothing in this tile can enter a list
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_d[w][t * 16 + 4 * (lane >> 4) + r][u * 16 + (lane & 15)] = dv[t][u][r];
+    // s_d belongs to this wave: a wave-scope fence orders its stores before
+    // the loads below (the LDS serves one wave's instructions in order)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (active) {
       // which of this lane's 16 candidates beat its current list
       const uint32_t lo = half * 16;
@@ -279,6 +316,7 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
           worst = bd[KM - 1];
         }
       }
+      s_w[w][half][ql] = worst;
     }
   }
   // lane ql merges the list of lane ql + 32
