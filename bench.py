@@ -39,6 +39,20 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_UPDATE_D2 = 80  # 8 (head+tail u32) + 16*d (H, T read+write) + 4*d*r (negatives), d=2, r=5
 
 
+def host_cpu() -> str:
+    """The host's CPU model and logical CPU count (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -712,8 +726,14 @@ def main() -> int:
             done += chunk
         dt = time.perf_counter() - t
         cpu_rate = (int(ost.counts[0]) - c0) / dt
+        # the graph build of the same kNN on the host (learn_manifold restated, OpenMP)
+        t = time.perf_counter()
+        O.learn_manifold(d_idx.to_host(), d_dst.to_host())
+        cpu_graph_s = time.perf_counter() - t
+        log(f"cpu baseline: learn_manifold {cpu_graph_s:.2f}s on {threads} threads (GPU {graph_s:.3f}s)")
         out["cpu_baseline"] = {"value": round(cpu_rate, 1), "unit": "edge-updates/s", "cores": threads,
-                               "kind": "port",
+                               "kind": "port", "host_cpu": host_cpu(),
+                               "graph_build_s": round(cpu_graph_s, 3),
                                "sample": f"same graph and init, Hogwild epochs 1..{done} of {epochs} "
                                          f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
         log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")
