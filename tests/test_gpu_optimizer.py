@@ -263,7 +263,7 @@ def test_opt_in_semantics_restore_cluster_structure(gpu, oracle):
 
 
 @pytest.mark.parametrize("semantics", [0, 1, 2, 3])
-@pytest.mark.parametrize("n_components", [2, 3])
+@pytest.mark.parametrize("n_components", [1, 2, 3, 5, 8])
 def test_default_epoch_bit_identical_to_sync_restatement(gpu, oracle, semantics, n_components):
     """The default epoch on a symmetric graph (row-owner synchronous) against
     the CPU restatement of the same epoch: embedding, schedule and counts
