@@ -322,3 +322,18 @@ def test_directed_graph_falls_back_and_keeps_quality(gpu, oracle):
     ce_cpu = E.fuzzy_cross_entropy(g.indptr(), g.indices(), g.data(), ost.embedding, a, b)
     print(f"[directed] CE gpu={ce_gpu:.1f} cpu={ce_cpu:.1f}", flush=True)
     assert abs(ce_gpu - ce_cpu) / ce_cpu < 0.05
+
+
+@pytest.mark.parametrize("params", [
+    {"negative_sample_rate": 0}, {"negative_sample_rate": 1}, {"negative_sample_rate": 10},
+    {"repulsion_strength": 0.5, "learning_rate": 0.3}, {"repulsion_strength": 2.0, "learning_rate": 1.7},
+])
+def test_default_epoch_bit_identical_across_sgd_parameters(gpu, oracle, params):
+    """The default epoch against its CPU restatement with other negative
+    sample rates (including none), repulsion strengths and learning rates."""
+    _, _, _, opt, ost = _setup(gpu, oracle, n=2000, total=40, seed=13, **params)
+    opt.step_epochs(40)
+    ost.step_epochs(40, "sync")
+    y = opt.embedding()
+    assert int((y.view(np.uint32) != ost.embedding.view(np.uint32)).sum()) == 0, params
+    assert opt.stats()[:2] == tuple(int(c) for c in ost.counts)
