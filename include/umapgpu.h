@@ -322,14 +322,16 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
  * at slot 0 (distance 0), then its k-1 nearest other points by ascending
  * distance (ties by index), distance = sqrt of sum((x_i - x_j)^2) summed in
  * f32 in dimension order.  data is n x dim row-major, all device pointers.
- * For dim >= 8 candidates are ranked on the matrix cores and re-ranked with
+ * For dim >= 4 candidates are ranked on the matrix cores and re-ranked with
  * that sum; an error bound proves each row exact or sends it to the scalar
- * kernel, so the output does not depend on the path. */
+ * kernel, so the output does not depend on the path.  1 <= dim <= 128: widths
+ * other than 2, 3, 8, 16, 20, 32, 64 and 128 are zero-padded to the next of
+ * them, which adds +0.0 to every sum and so changes no distance. */
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);
 /* Rows of the last umapgpu_knn_bruteforce_device call that the exactness
- * bound sent to the scalar kernel (0 for dim < 8, which only uses it). */
+ * bound sent to the scalar kernel (0 for dim <= 3, which only uses it). */
 UMAPGPU_API uint64_t umapgpu_knn_last_fallback_rows(void);
 /* ---- batched sweep (no reference counterpart: the reference steps one
  * Optimizer at a time, optimizer.rs:297).  Many independent Euclidean fits

@@ -45,6 +45,26 @@ def test_knn_matches_f32_restatement(gpu, dim, k):
     assert np.array_equal(gd.view(np.uint32), hd.view(np.uint32))
 
 
+@pytest.mark.parametrize("dim", [1, 5, 12, 50, 100, 127])
+def test_knn_any_width_up_to_128(gpu, dim):
+    """Widths without their own kernel are zero-padded to the next one; the
+    padding adds +0.0 to every sum, so the result equals the restatement at
+    the original width bit for bit."""
+    n, k = 900, 15
+    x, _ = gmm(n, dim, 4, seed=dim)
+    x[50:60] = x[70:80]  # exact ties
+    di, dd = gpu.knn_bruteforce_device(gpu.DeviceArray.from_host(x), k)
+    hi, hd = _host_knn_f32(x, k)
+    assert np.array_equal(di.to_host(), hi)
+    assert np.array_equal(dd.to_host().view(np.uint32), hd.view(np.uint32))
+
+
+def test_knn_rejects_wider_than_128(gpu):
+    x = np.zeros((40, 129), F)
+    with pytest.raises(gpu.UmapGpuError, match="1 <= dim <= 128"):
+        gpu.knn_bruteforce_device(gpu.DeviceArray.from_host(x), 5)
+
+
 @pytest.mark.parametrize("n", [1, 5, 20, 40])
 def test_knn_tiny_inputs(gpu, n):
     x = np.random.default_rng(n).normal(size=(n, 16)).astype(F)

@@ -449,9 +449,43 @@ void launch_mfma(const float* X, uint64_t n, uint64_t k, uint32_t* oi, float* od
 
 uint64_t g_knn_fallback_queries = 0;
 
+namespace {
+
+// Rows widened to `to` columns with zeros.  A zero column adds +0.0 to every
+// squared distance and norm, sums that are never -0, so the padded rows give
+// the same distances bit for bit.
+__global__ void pad_rows_kernel(const float* __restrict__ src, uint64_t n, uint32_t from, uint32_t to,
+                                float* __restrict__ dst) {
+  const uint64_t count = n * to;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / to, c = i - r * to;
+    dst[i] = c < from ? src[r * from + c] : 0.0f;
+  }
+}
+
+// The width whose kernel serves `dim` (the next instantiated one).
+uint32_t knn_padded_dim(uint64_t dim) {
+  static const uint32_t widths[] = {2, 3, 8, 16, 20, 32, 64, 128};
+  for (uint32_t w : widths)
+    if (dim <= w) return w;
+  return 0;
+}
+
+}  // namespace
+
 void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32_t* oi, float* od, hipStream_t s) {
   if (k < 1 || k > n) throw Error(Err::InvalidArgument, "kNN needs 1 <= k <= n");
   if (n >= 0xffffffffull) throw Error(Err::InvalidArgument, "kNN needs n < 2^32");
+  if (dim == 0 || dim > 128) throw Error(Err::Unsupported, "brute-force kNN supports 1 <= dim <= 128");
+  const uint32_t pd = knn_padded_dim(dim);
+  if (pd != dim) {
+    DevBuf padded(n * pd * 4);
+    pad_rows_kernel<<<(unsigned)std::min<uint64_t>(ceil_div(n * pd, 256), 1u << 16), 256, 0, s>>>(
+        X, n, (uint32_t)dim, pd, padded.as<float>());
+    UMAP_LAUNCH_CHECK();
+    knn_bruteforce(padded.as<float>(), n, pd, k, oi, od, s);  // synchronises before `padded` is freed
+    return;
+  }
   switch (dim) {
     case 2: launch_dim<2, true>(X, n, k, nullptr, n, oi, od, s); break;
     case 3: launch_dim<3, true>(X, n, k, nullptr, n, oi, od, s); break;
@@ -461,7 +495,7 @@ void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32
     case 32: launch_mfma<32>(X, n, k, oi, od, s); break;
     case 64: launch_mfma<64>(X, n, k, oi, od, s); break;
     case 128: launch_mfma<128>(X, n, k, oi, od, s); break;
-    default: throw Error(Err::Unsupported, "brute-force kNN supports dim in {2,3,8,16,20,32,64,128}");
+    default: throw Error(Err::Unsupported, "brute-force kNN supports 1 <= dim <= 128");
   }
   UMAP_HIP(hipStreamSynchronize(s));
 }
