@@ -324,9 +324,10 @@ UMAPGPU_API umapgpu_status umapgpu_synchronize(void);
  * f32 in dimension order.  data is n x dim row-major, all device pointers.
  * For dim >= 4 candidates are ranked on the matrix cores and re-ranked with
  * that sum; an error bound proves each row exact or sends it to the scalar
- * kernel, so the output does not depend on the path.  1 <= dim <= 128: widths
+ * kernel, so the output does not depend on the path.  Up to dim 128, widths
  * other than 2, 3, 8, 16, 20, 32, 64 and 128 are zero-padded to the next of
- * them, which adds +0.0 to every sum and so changes no distance. */
+ * them, which adds +0.0 to every sum and so changes no distance; wider rows
+ * (any dim, k <= 33) take a kernel that sums every pair directly. */
 UMAPGPU_API umapgpu_status umapgpu_knn_bruteforce_device(const float* data_device, uint64_t n, uint64_t dim,
                                                          uint64_t k, uint32_t* indices_device,
                                                          float* dists_device);

@@ -59,10 +59,18 @@ def test_knn_any_width_up_to_128(gpu, dim):
     assert np.array_equal(dd.to_host().view(np.uint32), hd.view(np.uint32))
 
 
-def test_knn_rejects_wider_than_128(gpu):
-    x = np.zeros((40, 129), F)
-    with pytest.raises(gpu.UmapGpuError, match="1 <= dim <= 128"):
-        gpu.knn_bruteforce_device(gpu.DeviceArray.from_host(x), 5)
+@pytest.mark.parametrize("dim,k", [(129, 15), (300, 1), (784, 15), (784, 30), (1000, 33)])
+def test_knn_wide_rows(gpu, dim, k):
+    """Widths above 128 (e.g. 784, MNIST's): one lane per query summing each
+    pair in dimension order, so equal to the restatement bit for bit."""
+    n = 700
+    x, _ = gmm(n, dim, 4, seed=dim + k)
+    x[5:9] = x[20:24]  # exact ties
+    di, dd = gpu.knn_bruteforce_device(gpu.DeviceArray.from_host(x), k)
+    hi, hd = _host_knn_f32(x, k)
+    assert np.array_equal(di.to_host(), hi)
+    assert np.array_equal(dd.to_host().view(np.uint32), hd.view(np.uint32))
+    assert gpu.knn_last_fallback_rows() == 0
 
 
 @pytest.mark.parametrize("n", [1, 5, 20, 40])

@@ -451,6 +451,87 @@ uint64_t g_knn_fallback_queries = 0;
 
 namespace {
 
+// Any width (dim > 128): one lane per query, 64 queries per block.
+// Candidates pass through LDS 32 at a time and dimensions 64 at a time; the
+// candidate chunk is stored dimension-major so one broadcast float4 read
+// serves four candidates.  Each pair's terms are added in dimension order
+// into its own running sum, which is the f32 sum that defines the output,
+// so no re-rank or bound is needed.
+constexpr int kWQ = 64;   // queries per block
+constexpr int kWC = 32;   // candidates per tile
+constexpr int kWD = 64;   // dimensions per chunk
+
+template <int KM>
+__global__ __launch_bounds__(kWQ) void knn_wide_kernel(const float* __restrict__ X, uint32_t n, uint32_t dim,
+                                                       uint32_t k, uint32_t* __restrict__ out_i,
+                                                       float* __restrict__ out_d) {
+  __shared__ float s_q[kWQ][kWD + 1];   // the block's queries, one chunk of dimensions
+  __shared__ float s_c[kWD][kWC + 4];   // a candidate chunk, dimension-major
+  const uint32_t q = blockIdx.x * kWQ + threadIdx.x;
+  const bool active = q < n;
+  float bd[KM];
+  uint32_t bi[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    bd[i] = INFINITY;
+    bi[i] = 0xffffffffu;
+  }
+  const uint32_t kk = k - 1;  // neighbours besides self
+  float worst = kk ? INFINITY : -INFINITY;  // k = 1: nothing enters
+  for (uint32_t c0 = 0; c0 < n; c0 += kWC) {
+    float acc[kWC];
+#pragma unroll
+    for (int c = 0; c < kWC; ++c) acc[c] = 0.0f;
+    for (uint32_t d0 = 0; d0 < dim; d0 += kWD) {
+      const uint32_t nd = (dim - d0) < (uint32_t)kWD ? (dim - d0) : (uint32_t)kWD;
+      __syncthreads();  // the previous chunk is consumed
+      for (uint32_t e = threadIdx.x; e < (uint32_t)(kWQ * kWD); e += kWQ) {
+        const uint32_t r = e / kWD, d = e - r * kWD;
+        const uint32_t qr = blockIdx.x * kWQ + r;
+        s_q[r][d] = (qr < n && d < nd) ? X[(uint64_t)qr * dim + d0 + d] : 0.0f;
+      }
+      for (uint32_t e = threadIdx.x; e < (uint32_t)(kWC * kWD); e += kWQ) {
+        const uint32_t c = e / kWD, d = e - c * kWD;
+        s_c[d][c] = (c0 + c < n && d < nd) ? X[(uint64_t)(c0 + c) * dim + d0 + d] : 0.0f;
+      }
+      __syncthreads();
+      for (uint32_t d = 0; d < nd; ++d) {
+        const float qd = s_q[threadIdx.x][d];
+#pragma unroll
+        for (int c = 0; c < kWC; c += 4) {
+          const knn_v4f cv = *reinterpret_cast<const knn_v4f*>(&s_c[d][c]);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float df = qd - cv[j];
+            acc[c + j] = acc[c + j] + df * df;
+          }
+        }
+      }
+    }
+    const uint32_t nc = (n - c0) < (uint32_t)kWC ? (n - c0) : (uint32_t)kWC;
+#pragma unroll
+    for (int c = 0; c < kWC; ++c) {
+      const uint32_t ci = c0 + (uint32_t)c;
+      if (active && (uint32_t)c < nc && ci != q && acc[c] < worst) {
+        list_insert<KM>(bd, bi, acc[c], ci);
+#pragma unroll
+        for (int i = 0; i < KM; ++i)
+          if ((uint32_t)i + 1 == kk) worst = bd[i];  // the kk-th entry (compile-time positions)
+      }
+    }
+  }
+  if (!active) return;
+  const uint64_t o = (uint64_t)q * k;
+  out_i[o] = q;
+  if (out_d) out_d[o] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < KM; ++i)
+    if ((uint32_t)i < kk) {
+      out_i[o + 1 + i] = bi[i];
+      if (out_d) out_d[o + 1 + i] = sqrtf(bd[i]);
+    }
+}
+
 // Rows widened to `to` columns with zeros.  A zero column adds +0.0 to every
 // squared distance and norm, sums that are never -0, so the padded rows give
 // the same distances bit for bit.
@@ -476,7 +557,19 @@ uint32_t knn_padded_dim(uint64_t dim) {
 void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32_t* oi, float* od, hipStream_t s) {
   if (k < 1 || k > n) throw Error(Err::InvalidArgument, "kNN needs 1 <= k <= n");
   if (n >= 0xffffffffull) throw Error(Err::InvalidArgument, "kNN needs n < 2^32");
-  if (dim == 0 || dim > 128) throw Error(Err::Unsupported, "brute-force kNN supports 1 <= dim <= 128");
+  if (dim == 0 || dim > 0xffffffffull) throw Error(Err::InvalidArgument, "kNN needs dim >= 1");
+  if (dim > 128) {
+    if (k > 33) throw Error(Err::Unsupported, "brute-force kNN supports k <= 33 for dim > 128");
+    const unsigned grid = (unsigned)ceil_div(n, kWQ);
+    g_knn_fallback_queries = 0;
+    if (k <= 17)
+      knn_wide_kernel<16><<<grid, kWQ, 0, s>>>(X, (uint32_t)n, (uint32_t)dim, (uint32_t)k, oi, od);
+    else
+      knn_wide_kernel<32><<<grid, kWQ, 0, s>>>(X, (uint32_t)n, (uint32_t)dim, (uint32_t)k, oi, od);
+    UMAP_LAUNCH_CHECK();
+    UMAP_HIP(hipStreamSynchronize(s));
+    return;
+  }
   const uint32_t pd = knn_padded_dim(dim);
   if (pd != dim) {
     DevBuf padded(n * pd * 4);
@@ -495,7 +588,7 @@ void knn_bruteforce(const float* X, uint64_t n, uint64_t dim, uint64_t k, uint32
     case 32: launch_mfma<32>(X, n, k, oi, od, s); break;
     case 64: launch_mfma<64>(X, n, k, oi, od, s); break;
     case 128: launch_mfma<128>(X, n, k, oi, od, s); break;
-    default: throw Error(Err::Unsupported, "brute-force kNN supports 1 <= dim <= 128");
+    default: throw Error(Err::Unsupported, "no kNN kernel for this width");
   }
   UMAP_HIP(hipStreamSynchronize(s));
 }
