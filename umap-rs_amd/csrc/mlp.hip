@@ -39,6 +39,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kH = kMlpHidden;
@@ -352,15 +353,25 @@ __global__ __launch_bounds__(kThreads) void mlp_wgrad_kernel(WgradArgs a) {
     }
     return;
   }
-  // biases and W3: thread owns hidden unit j
+  // biases and W3: thread owns hidden unit j; the b3 sums (over the same
+  // rows, in the same ascending order) ride along, so no thread walks the
+  // slice a second time
   const uint32_t j = t;
-  float g1 = 0.0f, g2 = 0.0f, g3[kMlpMaxOut];
+  float g1 = 0.0f, g2 = 0.0f, g3[kMlpMaxOut], gb3[kMlpMaxOut];
 #pragma unroll
-  for (int o = 0; o < kMlpMaxOut; ++o) g3[o] = 0.0f;
+  for (int o = 0; o < kMlpMaxOut; ++o) g3[o] = gb3[o] = 0.0f;
+#pragma unroll 2
   for (uint32_t m = kb; m < ke; m += 8) {
     const u32x4 p1 = *(const u32x4*)(a.dh1t + (uint64_t)j * a.ldb + m);
     const u32x4 p2 = *(const u32x4*)(a.dh2t + (uint64_t)j * a.ldb + m);
     const u32x4 p3 = *(const u32x4*)(a.h2t + (uint64_t)j * a.ldb + m);
+    f32x4 dy[kMlpMaxOut][2];
+#pragma unroll
+    for (int o = 0; o < kMlpMaxOut; ++o)
+      if (o < (int)a.O) {
+        dy[o][0] = *(const f32x4*)(a.dyt + (uint64_t)o * a.ldb + m);
+        dy[o][1] = *(const f32x4*)(a.dyt + (uint64_t)o * a.ldb + m + 4);
+      }
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const float d1 = bf2f((uint16_t)(p1[e / 2] >> (16 * (e & 1))));
@@ -370,17 +381,19 @@ __global__ __launch_bounds__(kThreads) void mlp_wgrad_kernel(WgradArgs a) {
       g2 = g2 + d2;
 #pragma unroll
       for (int o = 0; o < kMlpMaxOut; ++o)
-        if (o < (int)a.O) g3[o] = g3[o] + a.dyt[(uint64_t)o * a.ldb + m + e] * hv;
+        if (o < (int)a.O) {
+          const float dv = dy[o][e / 4][e % 4];
+          g3[o] = g3[o] + dv * hv;
+          gb3[o] = gb3[o] + dv;
+        }
     }
   }
   out[a.off_b1 + j] = g1;
   out[a.off_b2 + j] = g2;
   for (uint32_t o = 0; o < a.O; ++o) out[a.off_w3 + (uint64_t)o * kH + j] = g3[o];
-  if (j < a.O) {
-    float g = 0.0f;
-    for (uint32_t m = kb; m < ke; ++m) g = g + a.dyt[(uint64_t)j * a.ldb + m];
-    out[a.off_b3 + j] = g;
-  }
+#pragma unroll
+  for (int o = 0; o < kMlpMaxOut; ++o)
+    if ((uint32_t)o == j && j < a.O) out[a.off_b3 + j] = gb3[o];
 }
 
 struct AdamArgs {
