@@ -768,15 +768,12 @@ constexpr uint32_t kFusedRowsLds = 1024;  // row pointers staged in LDS (else re
 // epoch, 128 852, 64 877-1122 (with the window size tuned for each).
 constexpr int kFusedBlock = 256;
 
-// Reads of another row and the owner's writes go through the device's
-// coherence point (agent-scope relaxed atomics: sc1 loads and stores), so a
-// row written by its owner on one XCD is seen by lanes on every other XCD in
+// Reads of another row's live tail (the modes without epoch-start copies) go
+// through the device's coherence point (agent-scope relaxed atomics: sc1
+// loads), so a tail changed on one XCD is seen by lanes on every other XCD in
 // the same epoch, as the reference's threads see each other's stores.
 __device__ inline float remote_load(const float* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void owner_store(float* p, float v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A row whose run goes on in the next chunk.
@@ -788,16 +785,20 @@ struct RowCarry {
   float t[D];          // tail value so far
 };
 
+// The owner's one write of row j at the end of its run.  Only the epoch with
+// copies calls it: nothing reads H or T until the kernel ends, so these are
+// plain stores, streamed (nontemporal) so that they do not displace the
+// copy T0 that the negative samples gather from L2.
 template <int D, bool SHARED>
 __device__ inline void store_row(float* H, float* T, uint32_t j, const float* h0, const float* dh, const float* t0,
                                  const float* t) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (SHARED) {
-      owner_store(H + (uint64_t)j * D + d, (h0[d] + dh[d]) + (t[d] - t0[d]));
+      __builtin_nontemporal_store((h0[d] + dh[d]) + (t[d] - t0[d]), H + (uint64_t)j * D + d);
     } else {
-      owner_store(H + (uint64_t)j * D + d, h0[d] + dh[d]);
-      owner_store(T + (uint64_t)j * D + d, t[d]);
+      __builtin_nontemporal_store(h0[d] + dh[d], H + (uint64_t)j * D + d);
+      __builtin_nontemporal_store(t[d], T + (uint64_t)j * D + d);
     }
   }
 }
@@ -1015,8 +1016,9 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
             if (TWIN && !SHARED) {  // every read from the epoch-start copy; HT0 rows are {H, T}
               h0[d] = HT0[(uint64_t)j * 2 * D + d];
               t0[d] = HT0[(uint64_t)j * 2 * D + D + d];
-              hk[d] = HT0[(uint64_t)k * 2 * D + d];
-              tk[d] = HT0[(uint64_t)k * 2 * D + D + d];
+              // the tail pair is read once: streamed, leaving L2 to T0
+              hk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + d);
+              tk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + D + d);
             } else if (TWIN) {  // one shared array: its copy is T0
               h0[d] = t0[d] = T0[(uint64_t)j * D + d];
               hk[d] = tk[d] = T0[(uint64_t)k * D + d];
