@@ -6,9 +6,9 @@
 // query rows and returns their k nearest other points, without the self slot;
 // points with a non-finite coordinate are never neighbours.
 //
-// One thread per query; a block's 256 queries sweep all points in 64-point
-// tiles staged in LDS, where every lane reads the same candidate at the same
-// time (LDS broadcast).  Squared distances are sum((q-c)^2) in f32, so no
+// One thread per query; a block's 256 queries sweep all points (or one span
+// of them) in 64-point tiles staged in LDS, where every lane reads the same
+// candidate at the same time (LDS broadcast).  Squared distances are sum((q-c)^2) in f32, so no
 // cancellation; the top list stays sorted in registers and is only touched
 // when some lane of the wave finds a closer point.
 #include <hip/hip_runtime.h>
@@ -26,13 +26,21 @@ namespace {
 
 constexpr int kQ = 256;   // queries per block
 constexpr int kTC = 64;   // candidates per LDS tile
+constexpr uint64_t kSplitBlocks = 2048;  // few queries: split the points until the launch has this many blocks
+constexpr uint64_t kSplitMin = 4096;     // ... but keep at least this many points per span
 
 // SELF: output slot 0 is the query itself and k-1 neighbours follow; else the
 // k nearest others.  queries == nullptr means query t is row t.
+// Block row y scans the points [y*span, (y+1)*span).  With part_d set, each
+// query's sorted list over its span (squared distances, kk entries) goes to
+// part_d/part_i[(y*nq + t)*kk ...] for knn_merge_kernel; otherwise span >= n
+// and the final row is written.
 template <int DIM, int KM, bool SELF>
 __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, uint32_t n, uint32_t k,
                                                  const uint32_t* __restrict__ queries, uint32_t nq,
-                                                 uint32_t* __restrict__ out_i, float* __restrict__ out_d) {
+                                                 uint32_t* __restrict__ out_i, float* __restrict__ out_d,
+                                                 uint32_t span, float* __restrict__ part_d,
+                                                 uint32_t* __restrict__ part_i) {
   __shared__ float tile[kTC * DIM];
   const uint32_t t = blockIdx.x * kQ + threadIdx.x;
   const uint32_t q = t < nq ? (queries ? queries[t] : t) : 0xffffffffu;
@@ -49,8 +57,10 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
   }
   const uint32_t kk = SELF ? k - 1 : k;  // neighbours besides self
   float worst = INFINITY;
-  for (uint32_t c0 = 0; c0 < n; c0 += kTC) {
-    const uint32_t nc = (n - c0) < (uint32_t)kTC ? (n - c0) : (uint32_t)kTC;
+  const uint32_t cb = (uint32_t)min<uint64_t>((uint64_t)blockIdx.y * span, n);
+  const uint32_t ce = (uint32_t)min<uint64_t>((uint64_t)cb + span, n);
+  for (uint32_t c0 = cb; c0 < ce; c0 += kTC) {
+    const uint32_t nc = (ce - c0) < (uint32_t)kTC ? (ce - c0) : (uint32_t)kTC;
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < nc * DIM; e += kQ) tile[e] = X[(uint64_t)c0 * DIM + e];
     __syncthreads();
@@ -90,7 +100,67 @@ __global__ __launch_bounds__(kQ) void knn_kernel(const float* __restrict__ X, ui
     }
   }
   if (t >= nq) return;
+  if (part_d) {
+    const uint64_t o = ((uint64_t)blockIdx.y * nq + t) * kk;
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+      if (i < (int)kk) {
+        part_d[o + i] = bd[i];
+        part_i[o + i] = bi[i];
+      }
+    return;
+  }
   const uint64_t o = (uint64_t)(SELF ? q : t) * k;  // SELF: row q of the n x k output
+  if (SELF) {
+    out_i[o] = q;
+    if (out_d) out_d[o] = 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < KM; ++i)
+    if (i < (int)kk) {
+      out_i[o + (SELF ? 1 : 0) + i] = bi[i];
+      if (out_d) out_d[o + (SELF ? 1 : 0) + i] = sqrtf(bd[i]);
+    }
+}
+
+// The spans' lists of one query merged by the same (distance, index) order:
+// each span's list is exactly its kk least pairs, so the kk least of their
+// union are the kk least of all points, and the rows equal a single sweep's.
+template <int KM, bool SELF>
+__global__ void knn_merge_kernel(const float* __restrict__ part_d, const uint32_t* __restrict__ part_i,
+                                 uint32_t spans, uint32_t nq, uint32_t k, const uint32_t* __restrict__ queries,
+                                 uint32_t* __restrict__ out_i, float* __restrict__ out_d) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const uint32_t q = queries ? queries[t] : t;
+  const uint32_t kk = SELF ? k - 1 : k;
+  float bd[KM];
+  uint32_t bi[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    bd[i] = INFINITY;
+    bi[i] = 0xffffffffu;
+  }
+  for (uint32_t y = 0; y < spans; ++y) {
+    const uint64_t o = ((uint64_t)y * nq + t) * kk;
+    for (uint32_t j = 0; j < kk; ++j) {
+      float nd = part_d[o + j];
+      uint32_t ni = part_i[o + j];
+#pragma unroll
+      for (int i = 0; i < KM; ++i) {
+        if (i < (int)kk) {
+          const bool sw = nd < bd[i] || (nd == bd[i] && ni < bi[i]);
+          const float td = bd[i];
+          const uint32_t ti = bi[i];
+          bd[i] = sw ? nd : td;
+          bi[i] = sw ? ni : ti;
+          nd = sw ? td : nd;
+          ni = sw ? ti : ni;
+        }
+      }
+    }
+  }
+  const uint64_t o = (uint64_t)(SELF ? q : t) * k;
   if (SELF) {
     out_i[o] = q;
     if (out_d) out_d[o] = 0.0f;
@@ -378,18 +448,45 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const float* __restrict__
   if (!proven) redo[atomicAdd(redo_count, 1u)] = myq;
 }
 
+// Few queries (the matrix-core producer's unproven rows, the evaluators'
+// samples) would leave most CUs idle, so the points are split into spans as
+// well, one block row each, and the spans' lists merged afterwards.
+template <int DIM, int KM, bool SELF>
+void launch_scalar(const float* X, uint64_t n, uint64_t k, const uint32_t* queries, uint64_t nq, uint32_t* oi,
+                   float* od, hipStream_t s) {
+  const uint64_t qblocks = ceil_div(nq, kQ);
+  uint64_t spans = std::min<uint64_t>(ceil_div(kSplitBlocks, qblocks), ceil_div(n, kSplitMin));
+  if (spans <= 1) {
+    knn_kernel<DIM, KM, SELF><<<dim3((unsigned)qblocks, 1), kQ, 0, s>>>(X, (uint32_t)n, (uint32_t)k, queries,
+                                                                         (uint32_t)nq, oi, od, (uint32_t)n, nullptr,
+                                                                         nullptr);
+    UMAP_LAUNCH_CHECK();
+    return;
+  }
+  const uint64_t span = ceil_div(ceil_div(n, spans), (uint64_t)kTC) * kTC;
+  spans = ceil_div(n, span);
+  const uint64_t kk = SELF ? k - 1 : k;
+  DevBuf pd(std::max<uint64_t>(spans * nq * kk, 1) * 4), pi(std::max<uint64_t>(spans * nq * kk, 1) * 4);
+  knn_kernel<DIM, KM, SELF><<<dim3((unsigned)qblocks, (unsigned)spans), kQ, 0, s>>>(
+      X, (uint32_t)n, (uint32_t)k, queries, (uint32_t)nq, oi, od, (uint32_t)span, pd.as<float>(),
+      pi.as<uint32_t>());
+  UMAP_LAUNCH_CHECK();
+  knn_merge_kernel<KM, SELF><<<(unsigned)ceil_div(nq, 256), 256, 0, s>>>(
+      pd.as<float>(), pi.as<uint32_t>(), (uint32_t)spans, (uint32_t)nq, (uint32_t)k, queries, oi, od);
+  UMAP_LAUNCH_CHECK();
+  UMAP_HIP(hipStreamSynchronize(s));  // the span lists are freed on return
+}
+
 template <int DIM, bool SELF>
 void launch_dim(const float* X, uint64_t n, uint64_t k, const uint32_t* queries, uint64_t nq, uint32_t* oi,
                 float* od, hipStream_t s) {
   if (nq == 0) return;
-  const unsigned grid = (unsigned)ceil_div(nq, kQ);
   if (k <= 16)
-    knn_kernel<DIM, 16, SELF><<<grid, kQ, 0, s>>>(X, (uint32_t)n, (uint32_t)k, queries, (uint32_t)nq, oi, od);
+    launch_scalar<DIM, 16, SELF>(X, n, k, queries, nq, oi, od, s);
   else if (k <= 32)
-    knn_kernel<DIM, 32, SELF><<<grid, kQ, 0, s>>>(X, (uint32_t)n, (uint32_t)k, queries, (uint32_t)nq, oi, od);
+    launch_scalar<DIM, 32, SELF>(X, n, k, queries, nq, oi, od, s);
   else
     throw Error(Err::Unsupported, "brute-force kNN supports k <= 32");
-  UMAP_LAUNCH_CHECK();
 }
 
 // Squared row norms (sum of squares in dimension order) and their maximum,
