@@ -434,7 +434,7 @@ UMAPGPU_API umapgpu_status umapgpu_pca_device(const float* data_device, uint64_t
                                               float* y_device, double* variances);
 
 /* ---- evaluators (no reference counterpart: the reference defines no quality
- * measure).  Used to judge embeddings and to time wall-to-target-loss. ---- */
+ * measure).  Used to compare embeddings and to time wall-to-target-loss. ---- */
 typedef struct umapgpu_ce_terms {
   double total;        /* attract + (n^2 - edges) / pairs * repulse_sum */
   double attract;      /* sum over graph edges of w log(w/v) + (1-w) log((1-w)/(1-v)) */

@@ -1,5 +1,5 @@
 // Embedding-quality evaluators on the device.  The reference defines no
-// quality measure; these are the ones the fit path is judged by, computed on
+// quality measure; these are the ones the fit path's quality is measured by, computed on
 // the GPU so that they stay cheap at millions of points:
 //
 // * fuzzy cross-entropy of an embedding against the symmetric fuzzy graph
