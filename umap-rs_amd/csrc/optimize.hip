@@ -1016,9 +1016,18 @@ __global__ __launch_bounds__(BT) void sgd_fused_kernel(
             if (TWIN && !SHARED) {  // every read from the epoch-start copy; HT0 rows are {H, T}
               h0[d] = HT0[(uint64_t)j * 2 * D + d];
               t0[d] = HT0[(uint64_t)j * 2 * D + D + d];
-              // the tail pair is read once: streamed, leaving L2 to T0
-              hk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + d);
-              tk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + D + d);
+              // One fit: the tail pair is streamed (nontemporal), since across
+              // a large copy its rows are rarely read twice and caching them
+              // only displaces T0, which the negative samples gather from.  A
+              // sweep's members are small (a window's tail pairs fall in one
+              // fit's few MB), and there caching them measured 7% faster.
+              if constexpr (!SWEEP) {
+                hk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + d);
+                tk[d] = __builtin_nontemporal_load(HT0 + (uint64_t)k * 2 * D + D + d);
+              } else {
+                hk[d] = HT0[(uint64_t)k * 2 * D + d];
+                tk[d] = HT0[(uint64_t)k * 2 * D + D + d];
+              }
             } else if (TWIN) {  // one shared array: its copy is T0
               h0[d] = t0[d] = T0[(uint64_t)j * D + d];
               hk[d] = tk[d] = T0[(uint64_t)k * D + d];
