@@ -645,7 +645,7 @@ def main() -> int:
     achieved = bpu * float(sum(updates)) / kernel_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": {"hogwild": "sgd_fused_kernel<2,u16,row-owner synchronous> (+ snapshot_kernel per epoch)",
+                "kernel": {"hogwild": "sgd_fused_kernel<2,u16,row-owner synchronous> (+ snapshot2_kernel per epoch)",
                            "hogwild_window_atomic": "sgd_fused_kernel<2,u16,per-edge atomic tail>",
                            "hogwild_edge_atomic": "active_edges_kernel + sgd_edges_kernel<2,u16,atomic tail>",
                            "hogwild_racy": "active_edges_kernel + sgd_edges_kernel<2,u16,racy tail>",
