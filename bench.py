@@ -390,6 +390,26 @@ def sweep_section(U, args):
     return res
 
 
+def copy_peak(U, mib: int = 1024, reps: int = 10) -> dict:
+    """Measured stream-copy bandwidth: `reps` back-to-back launches of the plain
+    device copy (read + write of `mib` MiB each), bytes moved / wall time."""
+    nb = mib << 20
+    src, dst = U.DeviceArray((nb,), np.uint8), U.DeviceArray((nb,), np.uint8)
+    U.debug_copy(src, dst, nb)
+    U.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        U.debug_copy(src, dst, nb)
+    U.synchronize()
+    dt = time.perf_counter() - t
+    src.free()
+    dst.free()
+    gbps = 2.0 * nb * reps / dt / 1e9
+    log(f"stream copy: {gbps:.0f} GB/s ({reps} x {mib} MiB read + written)")
+    return {"GBps": round(gbps, 1), "frac_of_peak": round(gbps / HBM_PEAK_GBS, 3),
+            "note": f"plain device copy, {reps} launches of {mib} MiB read + {mib} MiB written, wall clock"}
+
+
 def mlp_profile(U) -> int:
     """Learned-init network kernels at the 5M x 128 workload's shapes: 3
     training epochs over 200k rows (batch 4096) and 3 predictions of 5M rows
@@ -666,6 +686,7 @@ def main() -> int:
                                             "the per-edge schedule streams come on top"}
         log(f"gather bound: {bound_us:.1f} us per epoch ({r8 / 1e9:.1f} G/s 8-B rows, {r16 / 1e9:.1f} G/s 16-B rows), "
             f"epoch {roofline['avg_launch_us']:.1f} us")
+    roofline["measured_copy"] = copy_peak(U)
     traffic_file = os.path.join(REPO, "profiles", "sgd_traffic_c2.json")
     if os.path.exists(traffic_file) and args.mode == "hogwild" and (n, k, epochs) == (1_000_000, 15, 500):
         with open(traffic_file) as f:
