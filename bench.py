@@ -748,13 +748,16 @@ def main() -> int:
         dt = time.perf_counter() - t
         cpu_rate = (int(ost.counts[0]) - c0) / dt
         # the graph build of the same kNN on the host (learn_manifold restated, OpenMP)
+        h_idx, h_dst = d_idx.to_host(), d_dst.to_host()
+        cpu_stages = {}
         t = time.perf_counter()
-        O.learn_manifold(d_idx.to_host(), d_dst.to_host())
+        O.learn_manifold(h_idx, h_dst, timings=cpu_stages)
         cpu_graph_s = time.perf_counter() - t
         log(f"cpu baseline: learn_manifold {cpu_graph_s:.2f}s on {threads} threads (GPU {graph_s:.3f}s)")
         out["cpu_baseline"] = {"value": round(cpu_rate, 1), "unit": "edge-updates/s", "cores": threads,
                                "kind": "port", "host_cpu": host_cpu(),
                                "graph_build_s": round(cpu_graph_s, 3),
+                               "graph_stages_ms": {k: round(1e3 * v, 1) for k, v in cpu_stages.items()},
                                "sample": f"same graph and init, Hogwild epochs 1..{done} of {epochs} "
                                          f"(OpenMP restatement of the Rayon loop, {threads} threads), {dt:.1f}s"}
         log(f"cpu baseline: {cpu_rate:.3g} updates/s on {threads} threads")

@@ -167,8 +167,20 @@ class Manifold:
 
 
 def learn_manifold(knn_indices, knn_dists, n_neighbors=None, local_connectivity=1.0, set_op_mix_ratio=1.0,
-                   symmetrize=True, disconnection_distance=np.inf, min_dist=0.1, spread=1.0, a=None, b=None):
-    """Umap::learn_manifold (embedding.rs:149-240) without the validation panics."""
+                   symmetrize=True, disconnection_distance=np.inf, min_dist=0.1, spread=1.0, a=None, b=None,
+                   timings=None):
+    """Umap::learn_manifold (embedding.rs:149-240) without the validation panics.
+    timings (a dict), if given, receives each stage's wall seconds under the
+    reference's tracing span names."""
+    import time
+    clock = [time.perf_counter()]
+
+    def stage(name):
+        now = time.perf_counter()
+        if timings is not None:
+            timings[name] = now - clock[0]
+        clock[0] = now
+
     d = np.ascontiguousarray(knn_dists, dtype=np.float32)
     n, k = d.shape
     n_neighbors = k if n_neighbors is None else n_neighbors
@@ -176,12 +188,18 @@ def learn_manifold(knn_indices, knn_dists, n_neighbors=None, local_connectivity=
         aa, bb = np.float32(a), np.float32(b)
     else:
         aa, bb, _ = find_ab_params(spread, min_dist)
+    stage("find_ab_params")
     edges_removed = int(np.count_nonzero(d >= np.float32(disconnection_distance)))
+    stage("disconnection detection")
     sig, rho = smooth_knn_dist(d, n_neighbors, 64, local_connectivity, 1.0)
+    stage("smooth_knn_dist")
     ip, ix, dv = membership_csr(knn_indices, d, sig, rho, disconnection_distance)
+    stage("build_membership_csr")
     if symmetrize:
         ip, ix, dv = set_operations(ip, ix, dv, set_op_mix_ratio)
+    stage("set_operations")
     vd, _ = zero_rows(ip, dv)
+    stage("disconnected vertices")
     return Manifold(ip, ix, dv, sig, rho, n, aa, bb, edges_removed, vd)
 
 
